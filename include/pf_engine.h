@@ -1,0 +1,166 @@
+/*
+ * pf_engine.h -- C-ABI of the MI355X (gfx950) SSSP / A* engine.
+ *
+ * Plain C types only (no HIP or C++ types), so any host language can bind it
+ * (ctypes, cgo, JNI).  Every call returns PF_OK (0) or a negative pf_status;
+ * pf_last_error() returns a thread-local message for the last failure.  No
+ * call ever exits the process (the reference's CUDA_CHECK exits,
+ * src/cuda_check.cuh:9-17; here errors are status codes).
+ *
+ * The C++ class GpuSolver (include/gpusolver/gpu_sssp.h, the reference's
+ * include/gpu_sssp.h:97-120 interface) is a thin layer over these calls.
+ */
+#ifndef PF_ENGINE_H
+#define PF_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+typedef enum {
+    PF_OK = 0,
+    PF_ERR_INVALID = -1,        /* bad argument: k, src/dst range (ref gpu_sssp.cu:463-465) */
+    PF_ERR_TOO_LARGE = -2,      /* maxBatch*n >= 2^32 (ref gpu_sssp.cu:108) */
+    PF_ERR_NO_MEMORY = -3,      /* device memory estimate exceeds free HBM (ref gpu_sssp.cu:113-127) */
+    PF_ERR_HIP = -4,            /* a HIP runtime call failed */
+    PF_ERR_NOT_CONVERGED = -5,  /* step cap exceeded (ref gpu_sssp.cu:273,345,416) */
+    PF_ERR_NO_DEVICE = -6,
+    PF_ERR_STATE = -7           /* call not valid in the object's current state */
+} pf_status;
+
+/* ---- enums: same members and order as the reference's gpu_sssp.h ---------- */
+typedef enum { PF_ALGO_NAIVE = 0, PF_ALGO_FRONTIER = 1, PF_ALGO_DELTA = 2 } pf_algo;       /* GpuAlgo  gpu_sssp.h:11-15 */
+typedef enum { PF_MAP_THREAD = 0, PF_MAP_WAVE = 1, PF_MAP_HYBRID = 2 } pf_mapping;       /* Mapping  gpu_sssp.h:18-22 (Warp -> 64-lane wave) */
+typedef enum { PF_ATOMIC_MIN_RBA = 0, PF_ATOMIC_CAS_RBA = 1, PF_ATOMIC_MIN_NOCHECK = 2 } pf_atomic; /* AtomicMode gpu_sssp.h:25-29 */
+typedef enum { PF_COMPACT_WAVE_ATOMIC = 0, PF_COMPACT_PREFIX_SCAN = 1 } pf_compaction;   /* Compaction gpu_sssp.h:32-35 */
+/* Who runs the bucket loop (no reference counterpart: the reference always
+ * drives it from the host, gpu_sssp.cu:390-455). */
+typedef enum { PF_DRIVE_HOST = 0, PF_DRIVE_GRAPH = 1, PF_DRIVE_PERSISTENT = 2 } pf_drive;
+/* A* heuristic.  SCALED = the reference's heuristicValue (include/heuristic.h:19-25);
+ * FLOOR_EUCLID = floor(Euclidean distance) on integer coordinates. */
+typedef enum { PF_H_NONE = 0, PF_H_SCALED = 1, PF_H_FLOOR_EUCLID = 2 } pf_heuristic;
+
+/* GpuOptions (gpu_sssp.h:42-52) plus the MI355X knobs. */
+typedef struct {
+    int32_t algo;          /* pf_algo, default PF_ALGO_DELTA */
+    int32_t mapping;       /* pf_mapping, default PF_MAP_HYBRID */
+    int32_t atomic;        /* pf_atomic, default PF_ATOMIC_MIN_RBA */
+    int32_t compaction;    /* pf_compaction, default PF_COMPACT_WAVE_ATOMIC (Frontier only) */
+    uint32_t delta;        /* 0 = Davidson default 32*avgW/avgDeg */
+    int32_t block_size;    /* multiple of 64 in [64,1024]; otherwise 256 */
+    uint32_t sync_every;   /* host drive: launches per host sync (default 8) */
+    int32_t drive;         /* pf_drive, default PF_DRIVE_HOST */
+    uint32_t wave_cutoff;  /* hybrid: degree >= this -> one wave per vertex (0 = 64) */
+    uint32_t block_cutoff; /* hybrid: degree >= this -> whole workgroup (0 = block_size) */
+    uint32_t grid_blocks;  /* workgroups per step launch (0 = automatic) */
+    int32_t heuristic;     /* pf_heuristic for A*; default PF_H_SCALED */
+    uint32_t max_steps;    /* 0 = automatic cap */
+    uint32_t reserved[3];
+} pf_options;
+
+/* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */
+typedef struct {
+    uint64_t edges_scanned, atomics_issued, updates, pushes;
+    uint32_t iterations, host_syncs, launches, buckets, delta, steps;
+    double ms;       /* init + loop (hipEvents on the engine stream) */
+    double loop_ms;  /* the step launches only (after init and seeding) */
+} pf_stats;
+
+/* Graph handed to the engine: CSR as built by the reference's buildCSR
+ * (graph.h:24-40).  Rows need not be sorted when flags has
+ * PF_GRAPH_SORT_ROWS: the engine then sorts each row by (w, dst) on the host
+ * before upload. */
+#define PF_GRAPH_SORT_ROWS 1u
+typedef struct {
+    uint32_t n, m;
+    const uint32_t* row_offsets; /* n+1 */
+    const uint32_t* col_indices; /* m */
+    const uint32_t* weights;     /* m, >= 1 */
+    const double* x;             /* n, or NULL */
+    const double* y;             /* n, or NULL */
+    double h_scale;              /* admissible scale of the SCALED heuristic */
+    double avg_w, avg_deg;       /* for autoDelta; 0 = computed */
+    uint32_t flags;
+} pf_graph;
+
+typedef struct pf_solver pf_solver;
+
+typedef struct {
+    int32_t ok;
+    char name[256];
+    char arch[64];
+    int32_t cu_count, clock_mhz, mem_clock_mhz, bus_width, max_threads_per_cu, wave_size;
+    int32_t driver_version, runtime_version;
+    uint64_t global_mem, lds_per_block, l2_bytes;
+    double peak_bandwidth_gbs;
+} pf_device_info;
+
+/* ---- library / device ---------------------------------------------------- */
+const char* pf_last_error(void);
+const char* pf_version(void);
+void pf_default_options(pf_options* o);
+int pf_device_query(int device, pf_device_info* out);    /* gpuQueryDevice gpu_device.cu:39-71 */
+int pf_warmup(void);                                      /* gpuWarmup gpu_device.cu:73-77 */
+/* Host->device CSR transfer study (gpuTransferStudy gpu_device.cu:82-160):
+ * ms_out = {pageable, pinned, pinned + 3 streams}. */
+int pf_transfer_study(const pf_graph* g, int reps, double* bytes_out, double* ms_out);
+
+/* ---- solver ---------------------------------------------------------------- */
+/* GpuSolver::GpuSolver(g, maxBatch) (gpu_solver ctor, gpu_sssp.cu:107-188). */
+int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out);
+void pf_solver_destroy(pf_solver* s);                     /* ~GpuSolver gpu_sssp.cu:190-200 */
+double pf_solver_upload_ms(const pf_solver* s);           /* uploadMs() gpu_sssp.h:105 */
+uint64_t pf_solver_device_bytes(const pf_solver* s);      /* deviceBytes() gpu_sssp.h:106 */
+uint32_t pf_solver_auto_delta(const pf_solver* s);        /* autoDelta() gpu_sssp.h:107, gpu_sssp.cu:202-206 */
+uint32_t pf_solver_n(const pf_solver* s);
+uint32_t pf_solver_max_batch(const pf_solver* s);
+
+/* GpuSolver::sssp (gpu_sssp.cu:545).  dist/parent: n entries each, or NULL
+ * (= copyBack false). */
+int pf_sssp(pf_solver* s, uint32_t src, const pf_options* o, uint32_t* dist, uint32_t* parent,
+            pf_stats* st);
+
+/* GpuSolver::astar (gpu_sssp.cu:549).  path: path_cap entries; *path_len = 0
+ * when unreachable.  The returned path is the canonical one: every vertex's
+ * parent is its smallest-index tight predecessor. */
+int pf_astar(pf_solver* s, uint32_t src, uint32_t dst, const pf_options* o,
+             uint32_t* target_dist, uint32_t* path, uint32_t path_cap, uint32_t* path_len,
+             pf_stats* st);
+
+/* GpuSolver::multiSource (gpu_sssp.cu:553-569): K sources over one shared
+ * frontier indexed k*n+v.  dist/parent: k*n entries (row k = source k) or NULL. */
+int pf_multi_source(pf_solver* s, const uint32_t* srcs, uint32_t k, const pf_options* o,
+                    uint32_t* dist, uint32_t* parent, pf_stats* st);
+
+/* Batched point-to-point A*: K (src,dst) pairs in one run, one slot each.
+ * paths: k*path_cap entries; path_lens/target_dists: k entries. */
+int pf_astar_batch(pf_solver* s, const uint32_t* srcs, const uint32_t* dsts, uint32_t k,
+                   const pf_options* o, uint32_t* target_dists, uint32_t* paths,
+                   uint32_t path_cap, uint32_t* path_lens, pf_stats* st);
+
+/* One query slot, several seeds at distance 0 (no parent).  With the
+ * disjoint union of many graphs and one seed per graph this runs every graph
+ * of a batch in a single run.  dist/parent: n entries or NULL. */
+int pf_multi_seed(pf_solver* s, const uint32_t* seeds, uint32_t nseeds, const pf_options* o,
+                  uint32_t* dist, uint32_t* parent, pf_stats* st);
+
+/* Statistics of the last run on device memory (for callers that skipped the
+ * copy back): number of vertices with dist < INF and the sum of their
+ * out-degrees (the reference's TEPS numerator, src/main.cpp:209-213), over
+ * all k slots of the last run. */
+int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs);
+
+/* Test hooks: the engine's own device computations for parity tests. */
+int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end); /* n entries */
+int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h); /* n entries */
+int pf_debug_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total);
+int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w); /* the uploaded CSR, m each */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

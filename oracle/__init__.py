@@ -1,0 +1,244 @@
+"""CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings of oracle/liboracle.so (the C restatement in oracle.c) and, when
+built, oracle/_ref/libpfref.so (the reference's own CPU sources compiled in
+place).  Imported only by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker or the timed CPU baseline; the product path
+(pathfinding_gpu_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+dblp = C.POINTER(C.c_double)
+
+H_NONE, H_SCALED, H_FLOOR_EUCLID = 0, 1, 2
+INF = 0xFFFFFFFF
+
+
+class _Stats(C.Structure):
+    _fields_ = [("minW", C.c_uint32), ("maxW", C.c_uint32), ("maxDeg", C.c_uint32), ("avgW", C.c_double),
+                ("avgDeg", C.c_double), ("hScale", C.c_double)]
+
+
+_SIG = {
+    "or_build_csr": (C.c_uint32, [C.c_uint32, C.c_size_t, u32p, u32p, u32p, u32p, u32p, u32p]),
+    "or_csr_stats": (None, [C.c_uint32, u32p, u32p, u32p, dblp, dblp, C.POINTER(_Stats)]),
+    "or_auto_delta": (C.c_uint32, [C.c_double, C.c_double]),
+    "or_light_end": (None, [C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
+    "or_heuristic": (C.c_uint32, [C.c_int, dblp, dblp, C.c_double, C.c_uint32, C.c_uint32]),
+    "or_dijkstra": (None, [C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, u64p]),
+    "or_dijkstra_multi_seed": (None, [C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32, u32p, u32p]),
+    "or_astar": (C.c_uint32, [C.c_uint32, u32p, u32p, u32p, dblp, dblp, C.c_double, C.c_int, C.c_uint32, C.c_uint32,
+                              C.c_int, u32p, C.c_uint32, u32p, u64p]),
+    "or_multi_source": (C.c_int, [C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32, C.c_int, u32p, u64p]),
+    "or_reconstruct_path": (C.c_uint32, [C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p, C.c_uint32]),
+    "or_canonical_parents": (None, [C.c_uint32, u32p, u32p, u32p, u32p, u32p]),
+    "or_bellman_ford": (None, [C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p]),
+    "or_verify_sssp": (None, [C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32,
+                              C.c_uint64, u64p, u64p]),
+    "or_validate_path": (C.c_int, [C.c_uint32, u32p, u32p, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_uint32]),
+    "or_reachable_arcs": (C.c_uint64, [C.c_uint32, u32p, u32p, u64p]),
+    "or_exclusive_scan": (C.c_uint64, [u32p, u32p, C.c_size_t]),
+    "or_next_bound": (C.c_uint64, [C.c_uint64, C.c_uint32]),
+    "or_time_dijkstra": (C.c_double, [C.c_uint32, u32p, u32p, u32p, C.c_uint32, C.c_int]),
+}
+
+_REF_SIG = {
+    "ref_build_csr": (C.c_void_p, [C.c_uint32, C.c_size_t, u32p, u32p, u32p, dblp, dblp]),
+    "ref_free": (None, [C.c_void_p]),
+    "ref_m": (C.c_uint32, [C.c_void_p]),
+    "ref_csr": (None, [C.c_void_p, u32p, u32p, u32p, u32p, dblp]),
+    "ref_dijkstra": (C.c_double, [C.c_void_p, C.c_uint32, u32p, u32p, u64p]),
+    "ref_astar": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, u32p, C.c_uint32, u32p, u64p, dblp]),
+    "ref_multi_source": (C.c_double, [C.c_void_p, u32p, C.c_uint32, C.c_int, u32p, u64p, C.POINTER(C.c_int)]),
+    "ref_heuristic": (C.c_uint32, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "ref_verify_sssp": (None, [C.c_void_p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, C.c_uint64, u64p]),
+    "ref_reconstruct_path": (C.c_uint32, [u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_uint32]),
+}
+
+
+def _p(a, t=C.c_uint32):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so and, when the reference sources exist, _ref/libpfref.so."""
+    out = subprocess.DEVNULL if quiet else None
+    subprocess.run(["make", "-C", _DIR, "all"], check=True, stdout=out)
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = os.path.join(_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _lib = C.CDLL(path)
+        for k, (r, a) in _SIG.items():
+            f = getattr(_lib, k)
+            f.restype, f.argtypes = r, a
+    return _lib
+
+
+def ref() -> Optional[C.CDLL]:
+    """The reference's own CPU code (oracle/_ref/libpfref.so), or None when not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_DIR, "_ref", "libpfref.so")
+        if not os.path.exists(path):
+            return None
+        _ref = C.CDLL(path)
+        for k, (r, a) in _REF_SIG.items():
+            f = getattr(_ref, k)
+            f.restype, f.argtypes = r, a
+    return _ref
+
+
+# ---------------------------------------------------------------------------
+# convenience wrappers over CSR arrays (row, col, w as uint32 numpy arrays)
+# ---------------------------------------------------------------------------
+def build_csr(n, src, dst, w):
+    src, dst, w = (np.ascontiguousarray(a, np.uint32) for a in (src, dst, w))
+    row = np.zeros(n + 1, np.uint32)
+    col = np.empty(max(len(src), 1), np.uint32)
+    wo = np.empty(max(len(src), 1), np.uint32)
+    m = lib().or_build_csr(n, len(src), _p(src), _p(dst), _p(w), _p(row), _p(col), _p(wo))
+    return row, col[:m].copy(), wo[:m].copy()
+
+
+def csr_stats(row, col, w, x=None, y=None) -> dict:
+    s = _Stats()
+    lib().or_csr_stats(len(row) - 1, _p(row), _p(col), _p(w), _p(x, C.c_double), _p(y, C.c_double), C.byref(s))
+    return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+def auto_delta(avg_w: float, avg_deg: float) -> int:
+    return lib().or_auto_delta(avg_w, avg_deg)
+
+
+def light_end(row, w, delta) -> np.ndarray:
+    out = np.empty(len(row) - 1, np.uint32)
+    lib().or_light_end(len(row) - 1, _p(row), _p(w), delta, _p(out))
+    return out
+
+
+def heuristic(kind, x, y, h_scale, v, t) -> int:
+    return lib().or_heuristic(kind, _p(x, C.c_double), _p(y, C.c_double), h_scale, v, t)
+
+
+def heuristic_all(kind, x, y, h_scale, t) -> np.ndarray:
+    return np.array([heuristic(kind, x, y, h_scale, v, t) for v in range(len(x))], np.uint32)
+
+
+def dijkstra(row, col, w, src):
+    """Returns (dist, parent_first, parent_canon, settled, edges)."""
+    n = len(row) - 1
+    d = np.empty(n, np.uint32)
+    pf = np.empty(n, np.uint32)
+    pc = np.empty(n, np.uint32)
+    st = np.zeros(2, np.uint64)
+    lib().or_dijkstra(n, _p(row), _p(col), _p(w), src, _p(d), _p(pf), _p(pc), _p(st, C.c_uint64))
+    return d, pf, pc, int(st[0]), int(st[1])
+
+
+def dijkstra_multi_seed(row, col, w, seeds):
+    n = len(row) - 1
+    s = np.ascontiguousarray(seeds, np.uint32)
+    d = np.empty(n, np.uint32)
+    pc = np.empty(n, np.uint32)
+    lib().or_dijkstra_multi_seed(n, _p(row), _p(col), _p(w), _p(s), len(s), _p(d), _p(pc))
+    return d, pc
+
+
+def astar(row, col, w, x, y, h_scale, hkind, src, dst, canonical=True):
+    """Returns (dist, path, settled, edges)."""
+    n = len(row) - 1
+    path = np.empty(n + 1, np.uint32)
+    ln = C.c_uint32(0)
+    st = np.zeros(2, np.uint64)
+    d = lib().or_astar(n, _p(row), _p(col), _p(w), _p(x, C.c_double), _p(y, C.c_double), h_scale, hkind, src, dst,
+                       1 if canonical else 0, _p(path), n + 1, C.byref(ln), _p(st, C.c_uint64))
+    return d, path[: ln.value].copy(), int(st[0]), int(st[1])
+
+
+def multi_source(row, col, w, srcs, threads=0, keep=True):
+    n = len(row) - 1
+    s = np.ascontiguousarray(srcs, np.uint32)
+    d = np.empty(len(s) * n, np.uint32) if keep else None
+    e = C.c_uint64(0)
+    thr = lib().or_multi_source(n, _p(row), _p(col), _p(w), _p(s), len(s), threads, _p(d), C.byref(e))
+    return (None if d is None else d.reshape(len(s), n)), e.value, thr
+
+
+def reconstruct_path(parent, src, dst) -> np.ndarray:
+    n = len(parent)
+    out = np.empty(n + 1, np.uint32)
+    ln = lib().or_reconstruct_path(n, _p(parent), src, dst, _p(out), n + 1)
+    return out[:ln].copy()
+
+
+def canonical_parents(row, col, w, dist) -> np.ndarray:
+    out = np.empty(len(row) - 1, np.uint32)
+    lib().or_canonical_parents(len(row) - 1, _p(row), _p(col), _p(w), _p(dist), _p(out))
+    return out
+
+
+def bellman_ford(row, col, w, src) -> np.ndarray:
+    out = np.empty(len(row) - 1, np.uint32)
+    lib().or_bellman_ford(len(row) - 1, _p(row), _p(col), _p(w), src, _p(out))
+    return out
+
+
+def verify_sssp(row, col, w, src, ref_dist, dist, parent, parent_canon=None, samples=64, seed=1):
+    """Returns dict with checks/errors of the layers dist, tree, paths, parent."""
+    checks = np.zeros(4, np.uint64)
+    errs = np.zeros(4, np.uint64)
+    lib().or_verify_sssp(len(row) - 1, _p(row), _p(col), _p(w), src, _p(ref_dist), _p(dist), _p(parent),
+                         _p(parent_canon), samples, seed, _p(checks, C.c_uint64), _p(errs, C.c_uint64))
+    names = ["dist", "tree", "paths", "parent"]
+    return {"checks": dict(zip(names, map(int, checks))), "errors": dict(zip(names, map(int, errs)))}
+
+
+def validate_path(row, col, w, path, src, dst, expected) -> bool:
+    p = np.ascontiguousarray(path, np.uint32)
+    return bool(lib().or_validate_path(len(row) - 1, _p(row), _p(col), _p(w), _p(p), len(p), src, dst, expected))
+
+
+def reachable_arcs(row, dist):
+    r = C.c_uint64(0)
+    a = lib().or_reachable_arcs(len(row) - 1, _p(row), _p(dist), C.byref(r))
+    return int(a), r.value
+
+
+def exclusive_scan(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty(len(a), np.uint32)
+    t = lib().or_exclusive_scan(_p(a), _p(out), len(a))
+    return out, int(t)
+
+
+def next_bound(key, delta) -> int:
+    return int(lib().or_next_bound(key, delta))
+
+
+def time_dijkstra(row, col, w, src, reps) -> float:
+    return lib().or_time_dijkstra(len(row) - 1, _p(row), _p(col), _p(w), src, reps)
+
+
+def bytes_alg(n, reached, arcs) -> int:
+    """Algorithmic HBM bytes of one SSSP instance: 8n (state init) + 16 per reached vertex + 16 per scanned arc."""
+    return 8 * n + 16 * reached + 16 * arcs

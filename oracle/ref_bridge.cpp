@@ -1,0 +1,130 @@
+// ref_bridge.cpp -- C entry points over the reference's own CPU code.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's src/cpu_sssp.cpp, src/graph.cpp, src/verify.cpp and
+// src/term.cpp (read in place, never copied) into oracle/_ref/libpfref.so.
+// Tests use it to pin the oracle's restatement against the real reference
+// functions, and bench.py's cpu_baseline leg may time cpuDijkstra with it.
+#include "cpu_sssp.h"
+#include "graph.h"
+#include "heuristic.h"
+#include "verify.h"
+
+#include <cstring>
+#include <stdexcept>
+
+extern "C" {
+
+// buildCSR (src/graph.cpp:23-104) on an edge list; returns an opaque graph.
+void* ref_build_csr(uint32_t n, size_t m_in, const uint32_t* src, const uint32_t* dst,
+                    const uint32_t* w, const double* x, const double* y) {
+    EdgeList el;
+    el.name = "bridge";
+    el.n = n;
+    el.src.assign(src, src + m_in);
+    el.dst.assign(dst, dst + m_in);
+    el.w.assign(w, w + m_in);
+    if (x && y) {
+        el.x.assign(x, x + n);
+        el.y.assign(y, y + n);
+    }
+    try {
+        return new CSRGraph(buildCSR(std::move(el)));
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void ref_free(void* g) { delete static_cast<CSRGraph*>(g); }
+
+uint32_t ref_m(void* gp) { return static_cast<CSRGraph*>(gp)->m; }
+
+// Copies row[n+1], col[m], w[m] out; stats = {minW, maxW, maxDeg} and
+// dstats = {avgW, avgDeg, hScale}.
+void ref_csr(void* gp, uint32_t* row, uint32_t* col, uint32_t* w, uint32_t* stats,
+             double* dstats) {
+    const CSRGraph& g = *static_cast<CSRGraph*>(gp);
+    std::memcpy(row, g.row_offsets.data(), (g.n + 1) * 4ull);
+    std::memcpy(col, g.col_indices.data(), g.m * 4ull);
+    std::memcpy(w, g.weights.data(), g.m * 4ull);
+    stats[0] = g.minW;
+    stats[1] = g.maxW;
+    stats[2] = g.maxDeg;
+    dstats[0] = g.avgW;
+    dstats[1] = g.avgDeg;
+    dstats[2] = g.hScale;
+}
+
+// cpuDijkstra (src/cpu_sssp.cpp:28-57); st = {settled, edgesScanned}; returns ms.
+double ref_dijkstra(void* gp, uint32_t src, uint32_t* dist, uint32_t* parent, uint64_t* st) {
+    SSSPResult r = cpuDijkstra(*static_cast<CSRGraph*>(gp), src);
+    if (dist) std::memcpy(dist, r.dist.data(), r.dist.size() * 4);
+    if (parent) std::memcpy(parent, r.parent.data(), r.parent.size() * 4);
+    if (st) {
+        st[0] = r.settled;
+        st[1] = r.edgesScanned;
+    }
+    return r.ms;
+}
+
+// cpuAStar (src/cpu_sssp.cpp:64-101); returns dist, writes the path.
+uint32_t ref_astar(void* gp, uint32_t src, uint32_t dst, int useH, uint32_t* path,
+                   uint32_t cap, uint32_t* len, uint64_t* st, double* ms) {
+    P2PResult r = cpuAStar(*static_cast<CSRGraph*>(gp), src, dst, useH != 0);
+    uint32_t l = (uint32_t)r.path.size();
+    if (l > cap) l = 0;
+    if (path && l) std::memcpy(path, r.path.data(), l * 4ull);
+    if (len) *len = l;
+    if (st) {
+        st[0] = r.settled;
+        st[1] = r.edgesScanned;
+    }
+    if (ms) *ms = r.ms;
+    return r.dist;
+}
+
+// cpuMultiSource (src/cpu_sssp.cpp:103-122); returns ms, threads via *thr.
+double ref_multi_source(void* gp, const uint32_t* srcs, uint32_t k, int threads,
+                        uint32_t* dist_out, uint64_t* edges, int* thr) {
+    const CSRGraph& g = *static_cast<CSRGraph*>(gp);
+    std::vector<u32> s(srcs, srcs + k);
+    MultiSourceCpuResult r = cpuMultiSource(g, s, threads, dist_out != nullptr);
+    if (dist_out)
+        for (uint32_t i = 0; i < k; ++i)
+            std::memcpy(dist_out + (size_t)i * g.n, r.dist[i].data(), g.n * 4ull);
+    if (edges) *edges = r.edgesScanned;
+    if (thr) *thr = r.threads;
+    return r.ms;
+}
+
+// heuristicValue (include/heuristic.h:19-25).
+uint32_t ref_heuristic(double xv, double yv, double xt, double yt, double scale) {
+    return heuristicValue(xv, yv, xt, yt, scale);
+}
+
+// verifySSSP (src/verify.cpp:54-99); out = {distMismatch, treeErrors, pathErrors,
+// distChecked, treeChecked, pathsChecked}.
+void ref_verify_sssp(void* gp, uint32_t src, const uint32_t* ref_dist, const uint32_t* dist,
+                     const uint32_t* parent, uint32_t samples, uint64_t seed, uint64_t* out) {
+    const CSRGraph& g = *static_cast<CSRGraph*>(gp);
+    std::vector<u32> rd(ref_dist, ref_dist + g.n), d(dist, dist + g.n), p(parent, parent + g.n);
+    VerifyReport r = verifySSSP(g, src, rd, d, p, samples, seed);
+    out[0] = r.distMismatch;
+    out[1] = r.treeErrors;
+    out[2] = r.pathErrors;
+    out[3] = r.distChecked;
+    out[4] = r.treeChecked;
+    out[5] = r.pathsChecked;
+}
+
+// reconstructPath (src/cpu_sssp.cpp:124-135).
+uint32_t ref_reconstruct_path(const uint32_t* parent, uint32_t n, uint32_t src, uint32_t dst,
+                              uint32_t* path, uint32_t cap) {
+    std::vector<u32> p(parent, parent + n);
+    std::vector<u32> r = reconstructPath(p, src, dst);
+    if (r.size() > cap) return 0;
+    std::memcpy(path, r.data(), r.size() * 4);
+    return (uint32_t)r.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""ctypes bindings of the native libraries (include/pf_engine.h, include/pf_graph.h).
+
+Loading fails loudly: there is no CPU fallback for any engine entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+dblp = C.POINTER(C.c_double)
+
+
+class PfOptions(C.Structure):
+    _fields_ = [
+        ("algo", C.c_int32), ("mapping", C.c_int32), ("atomic", C.c_int32), ("compaction", C.c_int32),
+        ("delta", C.c_uint32), ("block_size", C.c_int32), ("sync_every", C.c_uint32), ("drive", C.c_int32),
+        ("wave_cutoff", C.c_uint32), ("block_cutoff", C.c_uint32), ("grid_blocks", C.c_uint32),
+        ("heuristic", C.c_int32), ("max_steps", C.c_uint32), ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class PfStats(C.Structure):
+    _fields_ = [
+        ("edges_scanned", C.c_uint64), ("atomics_issued", C.c_uint64), ("updates", C.c_uint64),
+        ("pushes", C.c_uint64), ("iterations", C.c_uint32), ("host_syncs", C.c_uint32),
+        ("launches", C.c_uint32), ("buckets", C.c_uint32), ("delta", C.c_uint32), ("steps", C.c_uint32),
+        ("ms", C.c_double), ("loop_ms", C.c_double),
+    ]
+
+
+class PfGraph(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32), ("m", C.c_uint32), ("row_offsets", u32p), ("col_indices", u32p), ("weights", u32p),
+        ("x", dblp), ("y", dblp), ("h_scale", C.c_double), ("avg_w", C.c_double), ("avg_deg", C.c_double),
+        ("flags", C.c_uint32),
+    ]
+
+
+class PfDeviceInfo(C.Structure):
+    _fields_ = [
+        ("ok", C.c_int32), ("name", C.c_char * 256), ("arch", C.c_char * 64), ("cu_count", C.c_int32),
+        ("clock_mhz", C.c_int32), ("mem_clock_mhz", C.c_int32), ("bus_width", C.c_int32),
+        ("max_threads_per_cu", C.c_int32), ("wave_size", C.c_int32), ("driver_version", C.c_int32),
+        ("runtime_version", C.c_int32), ("global_mem", C.c_uint64), ("lds_per_block", C.c_uint64),
+        ("l2_bytes", C.c_uint64), ("peak_bandwidth_gbs", C.c_double),
+    ]
+
+
+# symbol -> (restype, argtypes)
+ENGINE_SYMBOLS = {
+    "pf_last_error": (C.c_char_p, []),
+    "pf_version": (C.c_char_p, []),
+    "pf_default_options": (None, [C.POINTER(PfOptions)]),
+    "pf_device_query": (C.c_int, [C.c_int, C.POINTER(PfDeviceInfo)]),
+    "pf_warmup": (C.c_int, []),
+    "pf_transfer_study": (C.c_int, [C.POINTER(PfGraph), C.c_int, dblp, dblp]),
+    "pf_solver_create": (C.c_int, [C.POINTER(PfGraph), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pf_solver_destroy": (None, [C.c_void_p]),
+    "pf_solver_upload_ms": (C.c_double, [C.c_void_p]),
+    "pf_solver_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "pf_solver_auto_delta": (C.c_uint32, [C.c_void_p]),
+    "pf_solver_n": (C.c_uint32, [C.c_void_p]),
+    "pf_solver_max_batch": (C.c_uint32, [C.c_void_p]),
+    "pf_sssp": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p, C.POINTER(PfStats)]),
+    "pf_astar": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(PfOptions), u32p, u32p, C.c_uint32, u32p,
+                           C.POINTER(PfStats)]),
+    "pf_multi_source": (C.c_int, [C.c_void_p, u32p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p,
+                                  C.POINTER(PfStats)]),
+    "pf_astar_batch": (C.c_int, [C.c_void_p, u32p, u32p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p, C.c_uint32,
+                                 u32p, C.POINTER(PfStats)]),
+    "pf_multi_seed": (C.c_int, [C.c_void_p, u32p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p,
+                                C.POINTER(PfStats)]),
+    "pf_last_reach": (C.c_int, [C.c_void_p, u64p, u64p]),
+    "pf_debug_light_end": (C.c_int, [C.c_void_p, C.c_uint32, u32p]),
+    "pf_debug_heuristic": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, u32p]),
+    "pf_debug_exclusive_scan": (C.c_int, [u32p, u32p, C.c_uint32, u32p]),
+    "pf_debug_sorted_rows": (C.c_int, [C.c_void_p, u32p, u32p]),
+}
+
+GRAPH_SYMBOLS = {
+    "pf_grid_road_arcs": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+    "pf_gen_grid_road": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int32, C.c_double, C.c_double, u32p, u32p,
+                                   u32p, dblp, dblp]),
+    "pf_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_uint64,
+                              u32p, u32p, u32p]),
+    "pf_build_csr": (C.c_int, [C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p, u32p]),
+    "pf_csr_stats": (C.c_int, [C.c_uint32, u32p, u32p, u32p, dblp, dblp, u32p, dblp]),
+    "pf_csr_union": (C.c_int, [C.c_uint32, u32p, u32p, C.POINTER(u32p), C.POINTER(u32p), C.POINTER(u32p), u32p, u32p,
+                               u32p, u32p]),
+    "pf_csr_save": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, dblp, dblp]),
+    "pf_csr_peek": (C.c_int, [C.c_char_p, u32p, u32p, u32p]),
+    "pf_csr_load": (C.c_int, [C.c_char_p, u32p, u32p, u32p, dblp, dblp]),
+    "pf_max_degree_vertex": (C.c_uint32, [C.c_uint32, u32p]),
+}
+
+
+def _load(name: str, symbols: dict) -> C.CDLL:
+    path = os.path.join(_PKG, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"native library {path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    for sym, (res, args) in symbols.items():
+        fn = getattr(lib, sym)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_engine = None
+_graph = None
+
+
+def engine() -> C.CDLL:
+    """The HIP engine library (libpfgpu.so)."""
+    global _engine
+    if _engine is None:
+        _engine = _load("libpfgpu.so", ENGINE_SYMBOLS)
+    return _engine
+
+
+def graphlib() -> C.CDLL:
+    """The host graph library (libpfgraph.so)."""
+    global _graph
+    if _graph is None:
+        _graph = _load("libpfgraph.so", GRAPH_SYMBOLS)
+    return _graph
+
+
+def ptr(a, ctype=C.c_uint32):
+    """Pointer to a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = engine().pf_last_error().decode(errors="replace")
+        raise RuntimeError(f"pathfinding engine error {rc}: {msg}")

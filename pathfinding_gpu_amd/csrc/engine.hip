@@ -1,0 +1,899 @@
+// engine.hip -- host side of the gfx950 SSSP / A* engine and its C-ABI.
+//
+// Reference counterpart: src/gpu_sssp.cu (GpuSolver::Impl: allocation and
+// CSR upload :107-188, autoDelta :202-206, ensureLightEnd :208-214,
+// runNaive/runFrontier/runDelta :257-456, run :461-534) and
+// src/gpu_device.cu (toString, gpuQueryDevice, gpuWarmup, gpuTransferStudy).
+// Differences by design:
+//   * the bucket / frontier loop decisions are made on the device by the
+//     step kernel's controller (pf_device.hpp); the host only launches steps
+//     and, depending on pf_options.drive, syncs every S launches, replays a
+//     captured hipGraph of S steps, or launches one persistent kernel;
+//   * arcs are stored interleaved {dst, w} (8 B per arc, one load);
+//   * errors are status codes, never exit().
+#include "pf_device.hpp"
+#include "../../include/pf_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace pf;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define PF_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(PF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+inline u32 divUp(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
+inline u64 maskWords(u64 bits) { return (bits + 31) / 32; }
+
+using StepFn = void (*)(Run, u32);
+
+template <bool A, int AM, int MAP>
+constexpr StepFn stepPtr() {
+    return &stepKernel<A, AM, MAP>;
+}
+
+StepFn stepTable[2][3][3] = {
+    {{stepPtr<false, 0, 0>(), stepPtr<false, 0, 1>(), stepPtr<false, 0, 2>()},
+     {stepPtr<false, 1, 0>(), stepPtr<false, 1, 1>(), stepPtr<false, 1, 2>()},
+     {stepPtr<false, 2, 0>(), stepPtr<false, 2, 1>(), stepPtr<false, 2, 2>()}},
+    {{stepPtr<true, 0, 0>(), stepPtr<true, 0, 1>(), stepPtr<true, 0, 2>()},
+     {stepPtr<true, 1, 0>(), stepPtr<true, 1, 1>(), stepPtr<true, 1, 2>()},
+     {stepPtr<true, 2, 0>(), stepPtr<true, 2, 1>(), stepPtr<true, 2, 2>()}},
+};
+
+struct GraphCacheEntry {
+    Run run;
+    StepFn fn;
+    u32 grid, bs, shm, S;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+}  // namespace
+
+struct pf_solver {
+    int dev = 0;
+    u32 n = 0, m = 0, K = 1;
+    bool hasCoords = false, intCoords = false;
+    double hScale = 0.0, avgW = 0.0, avgDeg = 0.0;
+    u32 lightDelta = 0;
+
+    u32* d_row = nullptr;
+    uint2* d_adj = nullptr;
+    u32* d_lightEnd = nullptr;
+    int2* d_xyi = nullptr;
+    double2* d_xyd = nullptr;
+
+    u64* d_state = nullptr;
+    u32* d_near[2] = {nullptr, nullptr};
+    u32* d_nearMask[2] = {nullptr, nullptr};
+    u32* d_far[2] = {nullptr, nullptr};
+    u32* d_farMask = nullptr;
+    u32* d_r = nullptr;
+    u32* d_rMask = nullptr;
+    u32* d_flags = nullptr;
+    u32* d_tileSums = nullptr;
+    u32 tileCap = 0;
+    u8* d_act[2] = {nullptr, nullptr};
+    u64* d_bound = nullptr;
+    u64* d_newBound = nullptr;
+    unsigned long long* d_minKey = nullptr;
+    u32* d_target = nullptr;
+    u32* d_slotDone = nullptr;
+    u32* d_src = nullptr;
+    Ctl* d_ctl = nullptr;
+    unsigned long long* d_stats = nullptr;
+    unsigned long long* d_reach = nullptr;
+    u32* d_seeds = nullptr;
+    u32 seedCap = 0;
+    u32* d_paths = nullptr;
+    u64 pathCap = 0;
+    u32* d_lens = nullptr;
+    u32* d_tdist = nullptr;
+
+    Ctl* h_ctl = nullptr;
+
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evLoop = nullptr;
+    double uploadMs = 0.0;
+    u64 bytes = 0;
+    int cuCount = 1, maxThreadsPerCU = 2048;
+    u32 lastK = 0;
+    std::vector<GraphCacheEntry> graphs;
+
+    template <class T>
+    int dalloc(T** p, u64 count) {
+        hipError_t e = hipMalloc((void**)p, std::max<u64>(1, count) * sizeof(T));
+        if (e != hipSuccess) return fail(PF_ERR_NO_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        bytes += count * sizeof(T);
+        return PF_OK;
+    }
+};
+
+namespace {
+
+// Estimate of the device bytes a solver needs (the reference's check,
+// src/gpu_sssp.cu:113-127, for this engine's buffers).
+u64 estimateBytes(u32 n, u32 m, u32 K, bool coords) {
+    const u64 Kn = (u64)K * n;
+    return (u64)(n + 1) * 4 + (u64)m * 8 + (u64)n * 4 + (coords ? (u64)n * 24 : 0)   // CSR, lightEnd, coords
+           + Kn * 8                                                                    // state
+           + 2 * Kn * 4 + 2 * Kn * 4 + Kn * 4                                          // near x2, far x2, R
+           + 4 * maskWords(Kn) * 4 + (u64)n * 2 + (u64)K * 64;                          // masks, naive flags, slots
+}
+
+void sortRows(u32 n, const u32* row, u32* col, u32* w) {
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (long long ui = 0; ui < (long long)n; ++ui) {
+        const u32 b = row[ui], e = row[ui + 1];
+        if (e - b < 2) continue;
+        std::vector<u64> tmp(e - b);
+        for (u32 k = b; k < e; ++k) tmp[k - b] = ((u64)w[k] << 32) | col[k];
+        std::sort(tmp.begin(), tmp.end());
+        for (u32 k = b; k < e; ++k) { w[k] = (u32)(tmp[k - b] >> 32); col[k] = (u32)tmp[k - b]; }
+    }
+}
+
+int setDev(const pf_solver* s) {
+    PF_HIP(hipSetDevice(s->dev));
+    return PF_OK;
+}
+
+void sanitize(pf_options& o) {
+    if (o.block_size < 64 || o.block_size > 1024 || (o.block_size % 64) != 0) o.block_size = 256;
+    if (o.algo < 0 || o.algo > 2) o.algo = PF_ALGO_DELTA;
+    if (o.mapping < 0 || o.mapping > 2) o.mapping = PF_MAP_HYBRID;
+    if (o.atomic < 0 || o.atomic > 2) o.atomic = PF_ATOMIC_MIN_RBA;
+    if (o.drive < 0 || o.drive > 2) o.drive = PF_DRIVE_HOST;
+    if (o.sync_every == 0) o.sync_every = 1;
+}
+
+int ensureLightEnd(pf_solver* s, u32 delta) {
+    if (delta == s->lightDelta) return PF_OK;
+    hipLaunchKernelGGL(lightEndKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, s->d_row, s->d_adj, s->n, delta,
+                       s->d_lightEnd);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipStreamSynchronize(s->s));
+    s->lightDelta = delta;
+    return PF_OK;
+}
+
+u32 autoDelta(const pf_solver* s) {
+    const double d = 32.0 * s->avgW / std::max(1.0, s->avgDeg);   // src/gpu_sssp.cu:204-205
+    return (u32)std::max<long long>(1, std::llround(d));
+}
+
+// One run request: seeds are element ids k*n+v; targets[k] for A*.
+struct RunSpec {
+    const u32* seeds = nullptr;
+    u32 nseeds = 0;
+    u32 k = 1;
+    const u32* targets = nullptr;
+    bool astar = false;
+    pf_options o{};
+};
+
+u32 defaultGrid(const pf_solver* s, u32 bs, u64 items) {
+    const u32 perCU = std::max(1, s->maxThreadsPerCU / (int)bs);
+    const u32 g = (u32)s->cuCount * perCU;
+    return std::max(1u, std::min<u32>(g, divUp(std::max<u64>(items, 1), bs)));
+}
+
+u32 inputSize(const Ctl& c, u32 n, u32 numTiles, u32 bs) {
+    switch (c.phase) {
+        case PH_LIGHT: case PH_FRONTIER: case PH_SCAN_RELAX: return divUp(c.cnt[C_NEAR0 + c.in], bs);
+        case PH_HEAVY: return divUp(c.cnt[C_R], bs);
+        case PH_FARMIN: case PH_PART: return divUp(c.cnt[C_FAR0 + c.fcur], bs);
+        case PH_SCAN_TILES: case PH_SCAN_SCATTER: return numTiles;
+        case PH_NAIVE: return divUp(n, bs);
+        default: return 1;
+    }
+}
+
+int launchStep(pf_solver* s, StepFn fn, u32 grid, u32 bs, u32 shm, Run& run, u32 persistent) {
+    void* args[] = {&run, &persistent};
+    if (persistent) {
+        PF_HIP(hipLaunchCooperativeKernel((const void*)fn, dim3(grid), dim3(bs), args, shm, s->s));
+    } else {
+        PF_HIP(hipLaunchKernel((const void*)fn, dim3(grid), dim3(bs), args, shm, s->s));
+    }
+    return PF_OK;
+}
+
+int getGraph(pf_solver* s, StepFn fn, u32 grid, u32 bs, u32 shm, u32 S, Run& run, hipGraphExec_t* out) {
+    for (auto& e : s->graphs) {
+        if (e.fn == fn && e.grid == grid && e.bs == bs && e.shm == shm && e.S == S &&
+            std::memcmp(&e.run, &run, sizeof(Run)) == 0) {
+            *out = e.exec;
+            return PF_OK;
+        }
+    }
+    GraphCacheEntry e;
+    e.run = run; e.fn = fn; e.grid = grid; e.bs = bs; e.shm = shm; e.S = S;
+    PF_HIP(hipStreamBeginCapture(s->s, hipStreamCaptureModeThreadLocal));
+    for (u32 i = 0; i < S; ++i) {
+        int rc = launchStep(s, fn, grid, bs, shm, run, 0);
+        if (rc != PF_OK) {
+            hipGraph_t g;
+            hipStreamEndCapture(s->s, &g);
+            return rc;
+        }
+    }
+    PF_HIP(hipStreamEndCapture(s->s, &e.graph));
+    PF_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+    s->graphs.push_back(e);
+    *out = e.exec;
+    return PF_OK;
+}
+
+int ensureSeeds(pf_solver* s, u32 count) {
+    if (count <= s->seedCap) return PF_OK;
+    if (s->d_seeds) { hipFree(s->d_seeds); s->d_seeds = nullptr; }
+    int rc = s->dalloc(&s->d_seeds, count);
+    if (rc) return rc;
+    s->seedCap = count;
+    return PF_OK;
+}
+
+int ensureFlags(pf_solver* s, u32 tileItems) {
+    const u64 Kn = (u64)s->K * s->n;
+    if (!s->d_flags) {
+        int rc = s->dalloc(&s->d_flags, Kn);
+        if (rc) return rc;
+        PF_HIP(hipMemset(s->d_flags, 0, Kn * 4));
+    }
+    const u32 nt = divUp(Kn, tileItems);
+    if (nt > s->tileCap) {
+        if (s->d_tileSums) hipFree(s->d_tileSums);
+        s->d_tileSums = nullptr;
+        int rc = s->dalloc(&s->d_tileSums, nt);
+        if (rc) return rc;
+        s->tileCap = nt;
+    }
+    return PF_OK;
+}
+
+int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    pf_options o = spec.o;
+    sanitize(o);
+    if (spec.k == 0 || spec.k > s->K) return fail(PF_ERR_INVALID, "invalid batch size");
+    if (spec.nseeds == 0) return fail(PF_ERR_INVALID, "no sources");
+    const u64 Kn = (u64)spec.k * s->n;
+    for (u32 i = 0; i < spec.nseeds; ++i)
+        if (spec.seeds[i] >= Kn) return fail(PF_ERR_INVALID, "source out of range");
+    if (spec.astar)
+        for (u32 i = 0; i < spec.k; ++i)
+            if (spec.targets[i] >= s->n) return fail(PF_ERR_INVALID, "target out of range");
+    const bool multiSeed = spec.nseeds > spec.k;
+    if ((spec.k > 1 || multiSeed) && o.algo == PF_ALGO_NAIVE) o.algo = PF_ALGO_DELTA;   // src/gpu_sssp.cu:466
+    const u32 delta = o.delta ? o.delta : autoDelta(s);
+    if (o.algo == PF_ALGO_DELTA) {
+        rc = ensureLightEnd(s, delta);
+        if (rc) return rc;
+    }
+    u32 hk = HK_NONE;
+    if (spec.astar) {
+        if (o.heuristic == PF_H_FLOOR_EUCLID) {
+            if (!s->intCoords) return fail(PF_ERR_INVALID, "FLOOR_EUCLID heuristic needs integer coordinates");
+            hk = HK_FLOOR_EUCLID;
+        } else if (o.heuristic == PF_H_SCALED && s->hasCoords && s->hScale > 0.0) {
+            hk = HK_SCALED;
+        }
+    }
+    const bool scanMode = o.algo == PF_ALGO_FRONTIER && o.compaction == PF_COMPACT_PREFIX_SCAN;
+    const u32 bs = (u32)o.block_size;
+    const u32 tileItems = bs * 16;
+    if (scanMode) {
+        rc = ensureFlags(s, tileItems);
+        if (rc) return rc;
+    }
+    rc = ensureSeeds(s, spec.nseeds);
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(s->d_seeds, spec.seeds, (size_t)spec.nseeds * 4, hipMemcpyHostToDevice, s->s));
+    if (spec.astar)
+        PF_HIP(hipMemcpyAsync(s->d_target, spec.targets, (size_t)spec.k * 4, hipMemcpyHostToDevice, s->s));
+
+    Run run{};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.state = s->d_state;
+    run.ctl = s->d_ctl;
+    run.stats = s->d_stats;
+    for (int i = 0; i < 2; ++i) {
+        run.nearQ[i] = s->d_near[i];
+        run.nearMask[i] = s->d_nearMask[i];
+        run.farQ[i] = s->d_far[i];
+        run.act[i] = s->d_act[i];
+    }
+    run.farMask = s->d_farMask;
+    run.rQ = s->d_r;
+    run.rMask = s->d_rMask;
+    run.flags = scanMode ? s->d_flags : nullptr;
+    run.tileSums = s->d_tileSums;
+    run.bound = s->d_bound;
+    run.newBound = s->d_newBound;
+    run.minKey = s->d_minKey;
+    run.target = spec.astar ? s->d_target : nullptr;
+    run.slotDone = s->d_slotDone;
+    run.hScale = s->hScale;
+    run.invN = 1.0 / (double)s->n;
+    run.K = spec.k;
+    run.delta = delta;
+    run.hkind = hk;
+    run.waveCutoff = o.wave_cutoff ? o.wave_cutoff : 64u;
+    run.blockCutoff = o.block_cutoff ? o.block_cutoff : bs;
+    run.tileItems = tileItems;
+    run.numTiles = divUp(Kn, tileItems);
+    run.total = (u32)Kn;
+
+    u32 startPhase = PH_LIGHT;
+    u32 mode = 0;
+    bool useBound = true;
+    if (o.algo == PF_ALGO_NAIVE) { startPhase = PH_NAIVE; mode = 1; useBound = false; }
+    else if (o.algo == PF_ALGO_FRONTIER) { startPhase = scanMode ? PH_SCAN_RELAX : PH_FRONTIER; useBound = false; }
+    const u64 capSteps = std::min<u64>(0xFFFFFF00ull, 16ull * Kn + 100000ull);
+    const u32 maxSteps = o.max_steps ? o.max_steps : (u32)capSteps;
+
+    const u32 shm = (bs / 64) * 64 * sizeof(GatherSlot);
+    const StepFn fn = stepTable[spec.astar ? 1 : 0][o.atomic][o.mapping];
+    u32 grid = o.grid_blocks ? o.grid_blocks : defaultGrid(s, bs, Kn);
+    u32 launches = 0, syncs = 0;
+
+    PF_HIP(hipEventRecord(s->ev0, s->s));   // ===== timed region (src/gpu_sssp.cu:478-502) =====
+    {
+        const u32 initGrid = std::min<u32>(divUp(Kn, 256), 8192u);
+        const u64 words = maskWords(Kn);
+        hipLaunchKernelGGL(initKernel, dim3(initGrid), dim3(256), 0, s->s, s->d_state, Kn, s->d_nearMask[0],
+                           s->d_nearMask[1], s->d_farMask, s->d_rMask, words, scanMode ? s->d_flags : nullptr, Kn,
+                           o.algo == PF_ALGO_NAIVE ? s->d_act[0] : nullptr, s->d_act[1], s->n);
+        PF_HIP(hipGetLastError());
+        const u32 sg = divUp(std::max(spec.nseeds, spec.k), 128);
+        hipLaunchKernelGGL(seedKernel, dim3(sg), dim3(128), 0, s->s, run, s->d_seeds, spec.nseeds, mode, startPhase,
+                           maxSteps, useBound ? 1u : 0u);
+        PF_HIP(hipGetLastError());
+    }
+    const u32 S = std::max(1u, o.sync_every);
+    PF_HIP(hipEventRecord(s->evLoop, s->s));
+    if (o.drive == PF_DRIVE_PERSISTENT) {
+        int nb = 0;
+        PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, (int)bs, shm));
+        const u32 maxCo = (u32)std::max(1, nb) * (u32)s->cuCount;
+        const u32 pg = std::max(1u, std::min(o.grid_blocks ? o.grid_blocks : maxCo, maxCo));
+        rc = launchStep(s, fn, pg, bs, shm, run, 1);
+        if (rc) return rc;
+        launches = 1;
+        PF_HIP(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->s));
+        PF_HIP(hipStreamSynchronize(s->s));
+        syncs = 1;
+    } else {
+        Ctl& hc = *s->h_ctl;
+        hc.phase = startPhase;
+        hc.in = 0; hc.fcur = 0;
+        for (int j = 0; j < C_COUNT; ++j) hc.cnt[j] = 0;
+        hc.cnt[C_NEAR0] = mode == 0 ? spec.nseeds : 0;
+        hipGraphExec_t exec = nullptr;
+        if (o.drive == PF_DRIVE_GRAPH) {
+            rc = getGraph(s, fn, grid, bs, shm, S, run, &exec);
+            if (rc) return rc;
+        }
+        while (true) {
+            if (exec) {
+                PF_HIP(hipGraphLaunch(exec, s->s));
+                launches += S;
+            } else {
+                for (u32 i = 0; i < S; ++i) {
+                    const u32 g = (S == 1 && !o.grid_blocks)
+                                      ? std::max(1u, std::min(defaultGrid(s, bs, Kn), inputSize(hc, s->n, run.numTiles, bs)))
+                                      : grid;
+                    rc = launchStep(s, fn, g, bs, shm, run, 0);
+                    if (rc) return rc;
+                    launches++;
+                }
+            }
+            PF_HIP(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->s));
+            PF_HIP(hipStreamSynchronize(s->s));
+            syncs++;
+            if (hc.phase == PH_DONE || hc.error) break;
+            if (hc.steps >= maxSteps) break;
+            if ((u64)launches > 2ull * maxSteps + 1000ull)
+                return fail(PF_ERR_NOT_CONVERGED, "step controller made no progress");
+        }
+    }
+    PF_HIP(hipEventRecord(s->ev1, s->s));
+    PF_HIP(hipEventSynchronize(s->ev1));
+    float ms = 0.f, loopMs = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    PF_HIP(hipEventElapsedTime(&loopMs, s->evLoop, s->ev1));
+    const Ctl& hc = *s->h_ctl;
+    if (hc.error & E_BARRIER) return fail(PF_ERR_HIP, "grid barrier wait exceeded its spin limit");
+    if (hc.error & E_NOT_CONVERGED || hc.phase != PH_DONE) return fail(PF_ERR_NOT_CONVERGED, "relaxation did not converge");
+    s->lastK = spec.k;
+    if (st) {
+        unsigned long long hs[5];
+        PF_HIP(hipMemcpy(hs, s->d_stats, sizeof(hs), hipMemcpyDeviceToHost));
+        st->edges_scanned = hs[0];
+        st->atomics_issued = hs[1];
+        st->updates = hs[2];
+        st->pushes = hs[3];
+        st->iterations = hc.iters;
+        st->host_syncs = syncs;
+        st->launches = launches;
+        st->buckets = o.algo == PF_ALGO_DELTA ? hc.buckets : 0;
+        st->delta = o.algo == PF_ALGO_DELTA ? delta : 0;
+        st->steps = hc.steps;
+        st->ms = ms;
+        st->loop_ms = loopMs;
+    }
+    return PF_OK;
+}
+
+int copyBack(pf_solver* s, u32 k, u32* dist, u32* parent) {
+    if (!dist && !parent) return PF_OK;
+    const u64 Kn = (u64)k * s->n;
+    std::vector<u64> hst(Kn);
+    PF_HIP(hipMemcpy(hst.data(), s->d_state, Kn * 8, hipMemcpyDeviceToHost));
+    for (u64 i = 0; i < Kn; ++i) {
+        if (dist) dist[i] = (u32)(hst[i] >> 32);
+        if (parent) parent[i] = (u32)hst[i];
+    }
+    return PF_OK;
+}
+
+int chasePaths(pf_solver* s, const u32* srcs, const u32* dsts, u32 k, u32* tdist, u32* paths, u32 cap, u32* lens) {
+    const u64 need = (u64)k * cap;
+    if (need > s->pathCap) {
+        if (s->d_paths) hipFree(s->d_paths);
+        s->d_paths = nullptr;
+        int rc = s->dalloc(&s->d_paths, need);
+        if (rc) return rc;
+        s->pathCap = need;
+    }
+    PF_HIP(hipMemcpyAsync(s->d_src, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s->s));
+    PF_HIP(hipMemcpyAsync(s->d_target, dsts, (size_t)k * 4, hipMemcpyHostToDevice, s->s));
+    hipLaunchKernelGGL(chaseKernel, dim3(divUp(k, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_src, s->d_target, k,
+                       s->d_paths, cap, s->d_lens, s->d_tdist);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpyAsync(tdist, s->d_tdist, (size_t)k * 4, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipMemcpyAsync(lens, s->d_lens, (size_t)k * 4, hipMemcpyDeviceToHost, s->s));
+    if (paths) PF_HIP(hipMemcpyAsync(paths, s->d_paths, need * 4, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    return PF_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+extern "C" {
+
+const char* pf_last_error(void) { return g_lastError.c_str(); }
+const char* pf_version(void) { return "pathfinding_gpu_amd 0.1 (gfx950)"; }
+
+void pf_default_options(pf_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->algo = PF_ALGO_DELTA;
+    o->mapping = PF_MAP_HYBRID;
+    o->atomic = PF_ATOMIC_MIN_RBA;
+    o->compaction = PF_COMPACT_WAVE_ATOMIC;
+    o->delta = 0;
+    o->block_size = 256;
+    o->sync_every = 8;
+    o->drive = PF_DRIVE_HOST;
+    o->heuristic = PF_H_SCALED;
+}
+
+int pf_device_query(int device, pf_device_info* out) {
+    std::memset(out, 0, sizeof(*out));
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device) {
+        (void)hipGetLastError();
+        return fail(PF_ERR_NO_DEVICE, "no HIP device");
+    }
+    hipDeviceProp_t p;
+    PF_HIP(hipGetDeviceProperties(&p, device));
+    out->ok = 1;
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->cu_count = p.multiProcessorCount;
+    out->clock_mhz = p.clockRate / 1000;
+    out->mem_clock_mhz = p.memoryClockRate / 1000;
+    out->bus_width = p.memoryBusWidth;
+    out->max_threads_per_cu = p.maxThreadsPerMultiProcessor;
+    out->wave_size = p.warpSize;
+    hipDriverGetVersion(&out->driver_version);
+    hipRuntimeGetVersion(&out->runtime_version);
+    out->global_mem = p.totalGlobalMem;
+    out->lds_per_block = p.sharedMemPerBlock;
+    out->l2_bytes = (uint64_t)p.l2CacheSize;
+    out->peak_bandwidth_gbs = 2.0 * (double)p.memoryClockRate * 1e3 * (p.memoryBusWidth / 8.0) / 1e9;
+    return PF_OK;
+}
+
+int pf_warmup(void) {
+    PF_HIP(hipSetDevice(0));
+    PF_HIP(hipFree(nullptr));
+    PF_HIP(hipDeviceSynchronize());
+    return PF_OK;
+}
+
+int pf_transfer_study(const pf_graph* g, int reps, double* bytes_out, double* ms_out) {
+    // gpuTransferStudy (src/gpu_device.cu:82-160): pageable vs pinned vs pinned + 3 streams
+    PF_HIP(hipSetDevice(0));
+    const size_t bRow = (size_t)(g->n + 1) * 4, bCol = (size_t)g->m * 4, bW = (size_t)g->m * 4;
+    const size_t total = bRow + bCol + bW;
+    *bytes_out = (double)total;
+    char* d = nullptr;
+    char* pin = nullptr;
+    PF_HIP(hipMalloc((void**)&d, total));
+    PF_HIP(hipHostMalloc((void**)&pin, total, hipHostMallocDefault));
+    std::memcpy(pin, g->row_offsets, bRow);
+    std::memcpy(pin + bRow, g->col_indices, bCol);
+    std::memcpy(pin + bRow + bCol, g->weights, bW);
+    hipStream_t st[3];
+    for (auto& x : st) PF_HIP(hipStreamCreate(&x));
+    reps = std::max(1, reps);
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto msSince = [](auto t0) { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+    double pageable = 0, pinned = 0, streams = 0;
+    for (int r = 0; r < reps; ++r) {
+        auto t0 = now();
+        PF_HIP(hipMemcpy(d, g->row_offsets, bRow, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(d + bRow, g->col_indices, bCol, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(d + bRow + bCol, g->weights, bW, hipMemcpyHostToDevice));
+        pageable += msSince(t0);
+        t0 = now();
+        PF_HIP(hipMemcpy(d, pin, total, hipMemcpyHostToDevice));
+        pinned += msSince(t0);
+        t0 = now();
+        PF_HIP(hipMemcpyAsync(d, pin, bRow, hipMemcpyHostToDevice, st[0]));
+        PF_HIP(hipMemcpyAsync(d + bRow, pin + bRow, bCol, hipMemcpyHostToDevice, st[1]));
+        PF_HIP(hipMemcpyAsync(d + bRow + bCol, pin + bRow + bCol, bW, hipMemcpyHostToDevice, st[2]));
+        for (auto& x : st) PF_HIP(hipStreamSynchronize(x));
+        streams += msSince(t0);
+    }
+    for (auto& x : st) hipStreamDestroy(x);
+    hipHostFree(pin);
+    hipFree(d);
+    ms_out[0] = pageable / reps;
+    ms_out[1] = pinned / reps;
+    ms_out[2] = streams / reps;
+    return PF_OK;
+}
+
+int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
+    *out = nullptr;
+    if (!g || !g->row_offsets || (g->m && (!g->col_indices || !g->weights)))
+        return fail(PF_ERR_INVALID, "null graph arrays");
+    const u32 K = std::max<u32>(1, max_batch);
+    if ((u64)K * g->n >= 0xFFFFFFFFull) return fail(PF_ERR_TOO_LARGE, "maxBatch * n must be < 2^32");
+    if (g->n == 0) return fail(PF_ERR_INVALID, "empty graph");
+    if (g->row_offsets[g->n] != g->m) return fail(PF_ERR_INVALID, "row_offsets[n] != m");
+    int dev = 0;
+    {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            return fail(PF_ERR_NO_DEVICE, "no HIP device");
+        }
+        PF_HIP(hipGetDevice(&dev));
+    }
+    const bool coords = g->x && g->y;
+    size_t freeB = 0, totalB = 0;
+    PF_HIP(hipMemGetInfo(&freeB, &totalB));
+    const u64 need = estimateBytes(g->n, g->m, K, coords);
+    if (need > freeB) {
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "Not enough GPU memory: need %.1f MB, free %.1f MB (reduce graph or batch)",
+                      need / 1048576.0, freeB / 1048576.0);
+        return fail(PF_ERR_NO_MEMORY, buf);
+    }
+    pf_solver* s = new pf_solver();
+    s->dev = dev;
+    s->n = g->n;
+    s->m = g->m;
+    s->K = K;
+    s->hScale = g->h_scale;
+    s->hasCoords = coords;
+    auto cleanupFail = [&](int rc) {
+        pf_solver_destroy(s);
+        return rc;
+    };
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "device properties"));
+    s->cuCount = p.multiProcessorCount;
+    s->maxThreadsPerCU = p.maxThreadsPerMultiProcessor;
+    if (hipStreamCreate(&s->s) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
+        hipEventCreate(&s->ev1) != hipSuccess || hipEventCreate(&s->evLoop) != hipSuccess)
+        return cleanupFail(fail(PF_ERR_HIP, "stream/event create"));
+
+    // statistics for autoDelta (src/graph.cpp:76-87)
+    if (g->avg_w > 0.0 && g->avg_deg > 0.0) {
+        s->avgW = g->avg_w;
+        s->avgDeg = g->avg_deg;
+    } else {
+        double sw = 0.0;
+        for (u32 k = 0; k < g->m; ++k) sw += g->weights[k];
+        s->avgW = g->m ? sw / g->m : 0.0;
+        s->avgDeg = (double)g->m / g->n;
+    }
+    int rc;
+    auto t0 = std::chrono::steady_clock::now();
+    // ---- CSR upload: interleaved {dst, w}, pinned staging, separate streams ----
+    if ((rc = s->dalloc(&s->d_row, (u64)s->n + 1)) || (rc = s->dalloc(&s->d_adj, s->m)) ||
+        (rc = s->dalloc(&s->d_lightEnd, s->n)))
+        return cleanupFail(rc);
+    {
+        const size_t bRow = (size_t)(s->n + 1) * 4, bAdj = (size_t)s->m * 8;
+        char* stage = nullptr;
+        if (hipHostMalloc((void**)&stage, bRow + bAdj, hipHostMallocDefault) != hipSuccess)
+            return cleanupFail(fail(PF_ERR_NO_MEMORY, "pinned staging"));
+        std::memcpy(stage, g->row_offsets, bRow);
+        uint2* adj = (uint2*)(stage + bRow);
+        std::vector<u32> col, w;
+        const u32* cp = g->col_indices;
+        const u32* wp = g->weights;
+        if (g->flags & PF_GRAPH_SORT_ROWS) {
+            col.assign(g->col_indices, g->col_indices + g->m);
+            w.assign(g->weights, g->weights + g->m);
+            sortRows(s->n, g->row_offsets, col.data(), w.data());
+            cp = col.data();
+            wp = w.data();
+        }
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < (long long)s->m; ++k) adj[k] = make_uint2(cp[k], wp[k] < 1u ? 1u : wp[k]);
+        hipStream_t st2;
+        if (hipStreamCreate(&st2) != hipSuccess) { hipHostFree(stage); return cleanupFail(fail(PF_ERR_HIP, "stream")); }
+        hipMemcpyAsync(s->d_row, stage, bRow, hipMemcpyHostToDevice, s->s);
+        hipMemcpyAsync(s->d_adj, stage + bRow, bAdj, hipMemcpyHostToDevice, st2);
+        hipError_t e1 = hipStreamSynchronize(s->s), e2 = hipStreamSynchronize(st2);
+        hipStreamDestroy(st2);
+        hipHostFree(stage);
+        if (e1 != hipSuccess || e2 != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "CSR upload"));
+    }
+    if (coords) {
+        bool integral = true;
+        for (u32 v = 0; v < s->n && integral; ++v) {
+            const double x = g->x[v], y = g->y[v];
+            if (x != std::floor(x) || y != std::floor(y) || std::fabs(x) > 1e9 || std::fabs(y) > 1e9) integral = false;
+        }
+        s->intCoords = integral;
+        std::vector<double2> xyd(s->n);
+        for (u32 v = 0; v < s->n; ++v) xyd[v] = make_double2(g->x[v], g->y[v]);
+        if ((rc = s->dalloc(&s->d_xyd, s->n))) return cleanupFail(rc);
+        if (hipMemcpy(s->d_xyd, xyd.data(), (size_t)s->n * 16, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanupFail(fail(PF_ERR_HIP, "coords upload"));
+        if (integral) {
+            std::vector<int2> xyi(s->n);
+            for (u32 v = 0; v < s->n; ++v) xyi[v] = make_int2((int)g->x[v], (int)g->y[v]);
+            if ((rc = s->dalloc(&s->d_xyi, s->n))) return cleanupFail(rc);
+            if (hipMemcpy(s->d_xyi, xyi.data(), (size_t)s->n * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanupFail(fail(PF_ERR_HIP, "coords upload"));
+        }
+    }
+    s->uploadMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+
+    // ---- working buffers (allocated once, src/gpu_sssp.cu:162-186) ----
+    const u64 Kn = (u64)K * s->n;
+    if ((rc = s->dalloc(&s->d_state, Kn))) return cleanupFail(rc);
+    for (int i = 0; i < 2; ++i) {
+        if ((rc = s->dalloc(&s->d_near[i], Kn)) || (rc = s->dalloc(&s->d_nearMask[i], maskWords(Kn))) ||
+            (rc = s->dalloc(&s->d_far[i], Kn)) || (rc = s->dalloc(&s->d_act[i], s->n)))
+            return cleanupFail(rc);
+    }
+    if ((rc = s->dalloc(&s->d_farMask, maskWords(Kn))) || (rc = s->dalloc(&s->d_r, Kn)) ||
+        (rc = s->dalloc(&s->d_rMask, maskWords(Kn))) || (rc = s->dalloc(&s->d_bound, K)) ||
+        (rc = s->dalloc(&s->d_newBound, K)) || (rc = s->dalloc(&s->d_minKey, K)) ||
+        (rc = s->dalloc(&s->d_target, K)) || (rc = s->dalloc(&s->d_slotDone, K)) ||
+        (rc = s->dalloc(&s->d_src, K)) || (rc = s->dalloc(&s->d_ctl, 1)) || (rc = s->dalloc(&s->d_stats, 5)) ||
+        (rc = s->dalloc(&s->d_reach, 2)) || (rc = s->dalloc(&s->d_lens, K)) || (rc = s->dalloc(&s->d_tdist, K)))
+        return cleanupFail(rc);
+    if (hipMemset(s->d_ctl, 0, sizeof(Ctl)) != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "memset"));
+    if (hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)
+        return cleanupFail(fail(PF_ERR_NO_MEMORY, "pinned ctl"));
+    if (hipDeviceSynchronize() != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "sync"));
+    *out = s;
+    return PF_OK;
+}
+
+void pf_solver_destroy(pf_solver* s) {
+    if (!s) return;
+    hipSetDevice(s->dev);
+    for (auto& e : s->graphs) {
+        hipGraphExecDestroy(e.exec);
+        hipGraphDestroy(e.graph);
+    }
+    void* ptrs[] = {s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+                    s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
+                    s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
+                    s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
+                    s->d_lens, s->d_tdist};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (s->h_ctl) hipHostFree(s->h_ctl);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->evLoop) hipEventDestroy(s->evLoop);
+    if (s->s) hipStreamDestroy(s->s);
+    delete s;
+}
+
+double pf_solver_upload_ms(const pf_solver* s) { return s->uploadMs; }
+uint64_t pf_solver_device_bytes(const pf_solver* s) { return s->bytes; }
+uint32_t pf_solver_auto_delta(const pf_solver* s) { return autoDelta(s); }
+uint32_t pf_solver_n(const pf_solver* s) { return s->n; }
+uint32_t pf_solver_max_batch(const pf_solver* s) { return s->K; }
+
+int pf_sssp(pf_solver* s, uint32_t src, const pf_options* o, uint32_t* dist, uint32_t* parent, pf_stats* st) {
+    if (!s || !o) return fail(PF_ERR_INVALID, "null argument");
+    if (src >= s->n) return fail(PF_ERR_INVALID, "source out of range");
+    RunSpec spec;
+    spec.seeds = &src;
+    spec.nseeds = 1;
+    spec.k = 1;
+    spec.o = *o;
+    int rc = doRun(s, spec, st);
+    if (rc) return rc;
+    return copyBack(s, 1, dist, parent);
+}
+
+int pf_multi_source(pf_solver* s, const uint32_t* srcs, uint32_t k, const pf_options* o, uint32_t* dist,
+                    uint32_t* parent, pf_stats* st) {
+    if (!s || !o || !srcs) return fail(PF_ERR_INVALID, "null argument");
+    if (k == 0 || k > s->K) return fail(PF_ERR_INVALID, "invalid batch size");
+    std::vector<u32> seeds(k);
+    for (u32 i = 0; i < k; ++i) {
+        if (srcs[i] >= s->n) return fail(PF_ERR_INVALID, "source out of range");
+        seeds[i] = i * s->n + srcs[i];
+    }
+    RunSpec spec;
+    spec.seeds = seeds.data();
+    spec.nseeds = k;
+    spec.k = k;
+    spec.o = *o;
+    int rc = doRun(s, spec, st);
+    if (rc) return rc;
+    return copyBack(s, k, dist, parent);
+}
+
+int pf_multi_seed(pf_solver* s, const uint32_t* seeds, uint32_t nseeds, const pf_options* o, uint32_t* dist,
+                  uint32_t* parent, pf_stats* st) {
+    if (!s || !o || !seeds) return fail(PF_ERR_INVALID, "null argument");
+    if (nseeds == 0 || nseeds > s->n) return fail(PF_ERR_INVALID, "invalid seed count");
+    for (u32 i = 0; i < nseeds; ++i)
+        if (seeds[i] >= s->n) return fail(PF_ERR_INVALID, "source out of range");
+    RunSpec spec;
+    spec.seeds = seeds;
+    spec.nseeds = nseeds;
+    spec.k = 1;
+    spec.o = *o;
+    int rc = doRun(s, spec, st);
+    if (rc) return rc;
+    return copyBack(s, 1, dist, parent);
+}
+
+int pf_astar_batch(pf_solver* s, const uint32_t* srcs, const uint32_t* dsts, uint32_t k, const pf_options* o,
+                   uint32_t* target_dists, uint32_t* paths, uint32_t path_cap, uint32_t* path_lens, pf_stats* st) {
+    if (!s || !o || !srcs || !dsts || !target_dists || !path_lens) return fail(PF_ERR_INVALID, "null argument");
+    if (k == 0 || k > s->K) return fail(PF_ERR_INVALID, "invalid batch size");
+    std::vector<u32> seeds(k);
+    for (u32 i = 0; i < k; ++i) {
+        if (srcs[i] >= s->n || dsts[i] >= s->n) return fail(PF_ERR_INVALID, "source/target out of range");
+        seeds[i] = i * s->n + srcs[i];
+    }
+    RunSpec spec;
+    spec.seeds = seeds.data();
+    spec.nseeds = k;
+    spec.k = k;
+    spec.targets = dsts;
+    spec.astar = true;
+    spec.o = *o;
+    int rc = doRun(s, spec, st);
+    if (rc) return rc;
+    return chasePaths(s, srcs, dsts, k, target_dists, paths, path_cap, path_lens);
+}
+
+int pf_astar(pf_solver* s, uint32_t src, uint32_t dst, const pf_options* o, uint32_t* target_dist, uint32_t* path,
+             uint32_t path_cap, uint32_t* path_len, pf_stats* st) {
+    if (!s || !o || !target_dist || !path_len) return fail(PF_ERR_INVALID, "null argument");
+    if (src >= s->n) return fail(PF_ERR_INVALID, "source out of range");
+    if (dst >= s->n) return fail(PF_ERR_INVALID, "target out of range");
+    return pf_astar_batch(s, &src, &dst, 1, o, target_dist, path, path_cap, path_len, st);
+}
+
+int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->lastK) return fail(PF_ERR_STATE, "no run yet");
+    const u64 total = (u64)s->lastK * s->n;
+    PF_HIP(hipMemsetAsync(s->d_reach, 0, 16, s->s));
+    hipLaunchKernelGGL(reachKernel, dim3(std::min<u32>(divUp(total, 256), 4096)), dim3(256), 0, s->s, s->d_state,
+                       s->d_row, s->n, total, s->d_reach);
+    PF_HIP(hipGetLastError());
+    unsigned long long h[2];
+    PF_HIP(hipMemcpyAsync(h, s->d_reach, 16, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    if (reached) *reached = h[0];
+    if (reachable_arcs) *reachable_arcs = h[1];
+    return PF_OK;
+}
+
+int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    s->lightDelta = 0;
+    rc = ensureLightEnd(s, delta);
+    if (rc) return rc;
+    PF_HIP(hipMemcpy(light_end, s->d_lightEnd, (size_t)s->n * 4, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (target >= s->n) return fail(PF_ERR_INVALID, "target out of range");
+    if (kind == PF_H_FLOOR_EUCLID && !s->intCoords) return fail(PF_ERR_INVALID, "no integer coordinates");
+    if (kind == PF_H_SCALED && !s->hasCoords) return fail(PF_ERR_INVALID, "no coordinates");
+    Run run{};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.hScale = s->hScale;
+    run.hkind = (u32)kind;
+    u32* d_h = nullptr;
+    PF_HIP(hipMalloc((void**)&d_h, (size_t)s->n * 4));
+    hipLaunchKernelGGL(heuristicKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, run, target, d_h);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_h, (size_t)s->n * 4, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+    hipFree(d_h);
+    if (e != hipSuccess) return fail(PF_ERR_HIP, hipGetErrorString(e));
+    return PF_OK;
+}
+
+int pf_debug_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total) {
+    u32 *d_in = nullptr, *d_out = nullptr, *d_t = nullptr;
+    PF_HIP(hipMalloc((void**)&d_in, std::max<size_t>(1, count) * 4));
+    PF_HIP(hipMalloc((void**)&d_out, std::max<size_t>(1, count) * 4));
+    PF_HIP(hipMalloc((void**)&d_t, 4));
+    PF_HIP(hipMemcpy(d_in, in, (size_t)count * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(exclusiveScanKernel, dim3(1), dim3(256), 0, 0, d_in, d_out, count, d_t);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpy(out, d_out, (size_t)count * 4, hipMemcpyDeviceToHost));
+    PF_HIP(hipMemcpy(total, d_t, 4, hipMemcpyDeviceToHost));
+    hipFree(d_in);
+    hipFree(d_out);
+    hipFree(d_t);
+    return PF_OK;
+}
+
+int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    std::vector<uint2> adj(s->m);
+    PF_HIP(hipMemcpy(adj.data(), s->d_adj, (size_t)s->m * 8, hipMemcpyDeviceToHost));
+    for (u32 k = 0; k < s->m; ++k) { col[k] = adj[k].x; w[k] = adj[k].y; }
+    return PF_OK;
+}
+
+}  // extern "C"

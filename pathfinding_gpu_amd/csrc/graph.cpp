@@ -1,0 +1,366 @@
+// graph.cpp -- host graph layer: workload generators, CSR construction,
+// disjoint union, binary cache (C-ABI in include/pf_graph.h) and the C++
+// mirror of the reference's graph API (include/gpusolver/graph.h).
+#include "../../include/pf_graph.h"
+#include "../../include/gpusolver/graph.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t GAMMA = 0x9E3779B97F4A7C15ull;
+
+// Draw j of stream t of a counter-based SplitMix64 generator.
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint64_t streamBase(uint64_t seed, uint64_t t) { return mix64(seed * GAMMA + t * 0xD1B54A32D192ED03ull + 1); }
+inline uint64_t draw(uint64_t base, uint64_t j) { return mix64(base + (j + 1) * GAMMA); }
+inline double unit(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }   // [0,1)
+
+}  // namespace
+
+extern "C" {
+
+uint64_t pf_grid_road_arcs(uint32_t rows, uint32_t cols) {
+    if (!rows || !cols) return 0;
+    return 2ull * ((uint64_t)rows * (cols - 1) + (uint64_t)cols * (rows - 1));
+}
+
+int pf_gen_grid_road(uint32_t rows, uint32_t cols, uint64_t seed, int32_t jitter, double wlo, double whi,
+                     uint32_t* src, uint32_t* dst, uint32_t* w, double* x, double* y) {
+    if (!rows || !cols || (uint64_t)rows * cols >= 0xFFFFFFFFull || jitter < 0) return -1;
+    const uint64_t n = (uint64_t)rows * cols;
+    const uint64_t bj = streamBase(seed, 1), bw = streamBase(seed, 2);
+    const uint64_t span = 2ull * (uint64_t)jitter + 1;
+#pragma omp parallel for schedule(static)
+    for (long long id = 0; id < (long long)n; ++id) {
+        const uint64_t r = (uint64_t)id / cols, c = (uint64_t)id % cols;
+        const long long jx = (long long)(draw(bj, 2ull * id) % span) - jitter;
+        const long long jy = (long long)(draw(bj, 2ull * id + 1) % span) - jitter;
+        x[id] = (double)(100ll * (long long)c + jx);
+        y[id] = (double)(100ll * (long long)r + jy);
+    }
+    const uint64_t nh = (uint64_t)rows * (cols - 1), nv = (uint64_t)cols * (rows - 1);
+#pragma omp parallel for schedule(static)
+    for (long long ri = 0; ri < (long long)(nh + nv); ++ri) {
+        uint32_t a, b;
+        if ((uint64_t)ri < nh) {
+            const uint64_t r = ri / (cols - 1), c = ri % (cols - 1);
+            a = (uint32_t)(r * cols + c);
+            b = a + 1;
+        } else {
+            const uint64_t q = ri - nh;
+            const uint64_t r = q / cols, c = q % cols;
+            a = (uint32_t)(r * cols + c);
+            b = a + cols;
+        }
+        const double dx = x[a] - x[b], dy = y[a] - y[b];
+        const double len = std::sqrt(dx * dx + dy * dy);
+        const double f = wlo + (whi - wlo) * unit(draw(bw, ri));
+        const double e = std::ceil(len * f);
+        const uint32_t wt = e < 1.0 ? 1u : (e > 4.0e9 ? 4000000000u : (uint32_t)e);
+        src[2 * ri] = a; dst[2 * ri] = b; w[2 * ri] = wt;
+        src[2 * ri + 1] = b; dst[2 * ri + 1] = a; w[2 * ri + 1] = wt;
+    }
+    return 0;
+}
+
+int pf_gen_rmat(uint32_t scale, uint32_t ef, double a, double b, double c, uint32_t wmax, uint64_t seed,
+                uint32_t* src, uint32_t* dst, uint32_t* w) {
+    if (scale == 0 || scale > 31 || wmax == 0) return -1;
+    const uint64_t m = (uint64_t)ef << scale;
+    const uint64_t base = streamBase(seed, 3);
+    const double ab = a + b, abc = a + b + c;
+#pragma omp parallel for schedule(static)
+    for (long long e = 0; e < (long long)m; ++e) {
+        const uint64_t j0 = (uint64_t)e * (scale + 1);
+        uint32_t u = 0, v = 0;
+        for (uint32_t l = 0; l < scale; ++l) {
+            const double r = unit(draw(base, j0 + l));
+            uint32_t bu = 0, bv = 0;
+            if (r < a) {
+            } else if (r < ab) {
+                bv = 1;
+            } else if (r < abc) {
+                bu = 1;
+            } else {
+                bu = 1;
+                bv = 1;
+            }
+            u = (u << 1) | bu;
+            v = (v << 1) | bv;
+        }
+        src[e] = u;
+        dst[e] = v;
+        w[e] = 1u + (uint32_t)(draw(base, j0 + scale) % wmax);
+    }
+    return 0;
+}
+
+int pf_build_csr(uint32_t n, uint64_t m_in, const uint32_t* src, const uint32_t* dst, const uint32_t* w,
+                 uint32_t* row, uint32_t* col, uint32_t* wout, uint32_t* m_out) {
+    std::vector<uint32_t> deg((size_t)n + 1, 0);
+    uint64_t keep = 0;
+    for (uint64_t i = 0; i < m_in; ++i) {
+        if (src[i] >= n || dst[i] >= n) return -1;
+        if (src[i] != dst[i]) { deg[src[i]]++; keep++; }
+    }
+    if (keep >= 0xFFFFFFFFull) return -2;
+    row[0] = 0;
+    for (uint32_t u = 0; u < n; ++u) row[u + 1] = row[u] + deg[u];
+    std::vector<uint32_t> cur(row, row + n);
+    for (uint64_t i = 0; i < m_in; ++i) {
+        const uint32_t s = src[i], d = dst[i];
+        if (s == d) continue;
+        const uint32_t p = cur[s]++;
+        col[p] = d;
+        wout[p] = std::max<uint32_t>(1u, w[i]);
+    }
+#pragma omp parallel
+    {
+        std::vector<uint64_t> tmp;
+#pragma omp for schedule(dynamic, 4096)
+        for (long long ui = 0; ui < (long long)n; ++ui) {
+            const uint32_t bg = row[ui], en = row[ui + 1];
+            if (en - bg < 2) continue;
+            tmp.resize(en - bg);
+            for (uint32_t k = bg; k < en; ++k) tmp[k - bg] = ((uint64_t)wout[k] << 32) | col[k];
+            std::sort(tmp.begin(), tmp.end());
+            for (uint32_t k = bg; k < en; ++k) {
+                wout[k] = (uint32_t)(tmp[k - bg] >> 32);
+                col[k] = (uint32_t)tmp[k - bg];
+            }
+        }
+    }
+    *m_out = (uint32_t)keep;
+    return 0;
+}
+
+int pf_csr_stats(uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w, const double* x,
+                 const double* y, uint32_t* iout, double* dout) {
+    const uint32_t m = row[n];
+    uint32_t mn = m ? 0xFFFFFFFFu : 0, mx = 0, md = 0;
+    double sw = 0.0;
+    for (uint32_t k = 0; k < m; ++k) {
+        mn = std::min(mn, w[k]);
+        mx = std::max(mx, w[k]);
+        sw += w[k];
+    }
+    for (uint32_t u = 0; u < n; ++u) md = std::max(md, row[u + 1] - row[u]);
+    double scale = std::numeric_limits<double>::infinity();
+    if (x && y) {
+        for (uint32_t u = 0; u < n; ++u)
+            for (uint32_t k = row[u]; k < row[u + 1]; ++k) {
+                const double dx = x[u] - x[col[k]], dy = y[u] - y[col[k]];
+                const double e = std::sqrt(dx * dx + dy * dy);
+                if (e > 1e-12) scale = std::min(scale, w[k] / e);
+            }
+    }
+    iout[0] = mn;
+    iout[1] = mx;
+    iout[2] = md;
+    dout[0] = m ? sw / m : 0.0;
+    dout[1] = n ? (double)m / n : 0.0;
+    dout[2] = (x && y && std::isfinite(scale)) ? scale : 0.0;
+    return 0;
+}
+
+int pf_csr_union(uint32_t count, const uint32_t* ns, const uint32_t* ms, const uint32_t* const* rows,
+                 const uint32_t* const* cols, const uint32_t* const* ws, uint32_t* voff, uint32_t* row,
+                 uint32_t* col, uint32_t* w) {
+    std::vector<uint64_t> eoff(count + 1, 0);
+    voff[0] = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint64_t nv = (uint64_t)voff[i] + ns[i];
+        eoff[i + 1] = eoff[i] + ms[i];
+        if (nv >= 0xFFFFFFFFull || eoff[i + 1] >= 0xFFFFFFFFull) return -2;
+        voff[i + 1] = (uint32_t)nv;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long i = 0; i < (long long)count; ++i) {
+        const uint32_t vo = voff[i];
+        const uint32_t eo = (uint32_t)eoff[i];
+        for (uint32_t u = 0; u < ns[i]; ++u) row[vo + u] = eo + rows[i][u];
+        for (uint32_t k = 0; k < ms[i]; ++k) {
+            col[eo + k] = vo + cols[i][k];
+            w[eo + k] = ws[i][k];
+        }
+    }
+    row[voff[count]] = (uint32_t)eoff[count];
+    return 0;
+}
+
+static const char kMagic[8] = {'P', 'F', 'C', 'S', 'R', '0', '0', '1'};
+
+int pf_csr_save(const char* path, const char* name, uint32_t n, uint32_t m, const uint32_t* row,
+                const uint32_t* col, const uint32_t* w, const double* x, const double* y) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return -1;
+    const uint32_t hasXY = (x && y) ? 1u : 0u;
+    const uint32_t nameLen = name ? (uint32_t)std::strlen(name) : 0u;
+    bool ok = std::fwrite(kMagic, 1, 8, f) == 8;
+    ok = ok && std::fwrite(&n, 4, 1, f) == 1 && std::fwrite(&m, 4, 1, f) == 1;
+    ok = ok && std::fwrite(&hasXY, 4, 1, f) == 1 && std::fwrite(&nameLen, 4, 1, f) == 1;
+    if (nameLen) ok = ok && std::fwrite(name, 1, nameLen, f) == nameLen;
+    ok = ok && std::fwrite(row, 4, (size_t)n + 1, f) == (size_t)n + 1;
+    ok = ok && std::fwrite(col, 4, m, f) == m && std::fwrite(w, 4, m, f) == m;
+    if (hasXY) ok = ok && std::fwrite(x, 8, n, f) == n && std::fwrite(y, 8, n, f) == n;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+static FILE* openHeader(const char* path, uint32_t* n, uint32_t* m, uint32_t* hasXY) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return nullptr;
+    char mg[8];
+    uint32_t nameLen = 0;
+    bool ok = std::fread(mg, 1, 8, f) == 8 && std::memcmp(mg, kMagic, 8) == 0;
+    ok = ok && std::fread(n, 4, 1, f) == 1 && std::fread(m, 4, 1, f) == 1 && std::fread(hasXY, 4, 1, f) == 1 &&
+         std::fread(&nameLen, 4, 1, f) == 1;
+    ok = ok && std::fseek(f, nameLen, SEEK_CUR) == 0;
+    if (!ok) {
+        std::fclose(f);
+        return nullptr;
+    }
+    return f;
+}
+
+int pf_csr_peek(const char* path, uint32_t* n, uint32_t* m, uint32_t* has_xy) {
+    FILE* f = openHeader(path, n, m, has_xy);
+    if (!f) return -1;
+    std::fclose(f);
+    return 0;
+}
+
+int pf_csr_load(const char* path, uint32_t* row, uint32_t* col, uint32_t* w, double* x, double* y) {
+    uint32_t n = 0, m = 0, hasXY = 0;
+    FILE* f = openHeader(path, &n, &m, &hasXY);
+    if (!f) return -1;
+    bool ok = std::fread(row, 4, (size_t)n + 1, f) == (size_t)n + 1;
+    ok = ok && std::fread(col, 4, m, f) == m && std::fread(w, 4, m, f) == m;
+    if (ok && hasXY && x && y) ok = std::fread(x, 8, n, f) == n && std::fread(y, 8, n, f) == n;
+    std::fclose(f);
+    return ok ? 0 : -1;
+}
+
+uint32_t pf_max_degree_vertex(uint32_t n, const uint32_t* row) {
+    uint32_t best = 0, bd = 0;
+    for (uint32_t u = 0; u < n; ++u)
+        if (row[u + 1] - row[u] > bd) { bd = row[u + 1] - row[u]; best = u; }
+    return best;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// C++ mirror of the reference's graph API (include/graph.h:24-72)
+// ---------------------------------------------------------------------------
+CSRGraph buildCSR(EdgeList&& el) {
+    CSRGraph g;
+    g.name = el.name;
+    g.n = el.n;
+    g.row_offsets.assign((size_t)g.n + 1, 0);
+    g.col_indices.resize(el.src.size());
+    g.weights.resize(el.src.size());
+    uint32_t m = 0;
+    if (pf_build_csr(g.n, el.src.size(), el.src.data(), el.dst.data(), el.w.data(), g.row_offsets.data(),
+                     g.col_indices.data(), g.weights.data(), &m) != 0)
+        throw std::runtime_error("buildCSR: invalid edge list or m >= 2^32");
+    g.m = m;
+    g.col_indices.resize(m);
+    g.weights.resize(m);
+    std::vector<uint32_t>().swap(el.src);
+    std::vector<uint32_t>().swap(el.dst);
+    std::vector<uint32_t>().swap(el.w);
+    g.x = std::move(el.x);
+    g.y = std::move(el.y);
+    uint32_t iout[3];
+    double dout[3];
+    pf_csr_stats(g.n, g.row_offsets.data(), g.col_indices.data(), g.weights.data(), g.hasCoords() ? g.x.data() : nullptr,
+                 g.hasCoords() ? g.y.data() : nullptr, iout, dout);
+    g.minW = iout[0];
+    g.maxW = iout[1];
+    g.maxDeg = iout[2];
+    g.avgW = dout[0];
+    g.avgDeg = dout[1];
+    g.hScale = dout[2];
+    return g;
+}
+
+EdgeList genGridRoadSpec(u32 rows, u32 cols, u64 seed, int jitter, double wlo, double whi) {
+    EdgeList el;
+    el.name = "grid_" + std::to_string(rows) + "x" + std::to_string(cols);
+    el.n = rows * cols;
+    const uint64_t m = pf_grid_road_arcs(rows, cols);
+    el.src.resize(m);
+    el.dst.resize(m);
+    el.w.resize(m);
+    el.x.resize(el.n);
+    el.y.resize(el.n);
+    if (pf_gen_grid_road(rows, cols, seed, jitter, wlo, whi, el.src.data(), el.dst.data(), el.w.data(), el.x.data(),
+                         el.y.data()) != 0)
+        throw std::runtime_error("genGridRoadSpec: bad size");
+    return el;
+}
+
+EdgeList genRMATSpec(u32 scale, u32 edgeFactor, double a, double b, double c, u32 maxW, u64 seed) {
+    EdgeList el;
+    el.name = "rmat_s" + std::to_string(scale) + "_ef" + std::to_string(edgeFactor);
+    el.n = 1u << scale;
+    const uint64_t m = (uint64_t)edgeFactor << scale;
+    el.src.resize(m);
+    el.dst.resize(m);
+    el.w.resize(m);
+    if (pf_gen_rmat(scale, edgeFactor, a, b, c, maxW, seed, el.src.data(), el.dst.data(), el.w.data()) != 0)
+        throw std::runtime_error("genRMATSpec: bad parameters");
+    return el;
+}
+
+bool saveBinary(const CSRGraph& g, const std::string& path) {
+    return pf_csr_save(path.c_str(), g.name.c_str(), g.n, g.m, g.row_offsets.data(), g.col_indices.data(),
+                       g.weights.data(), g.hasCoords() ? g.x.data() : nullptr,
+                       g.hasCoords() ? g.y.data() : nullptr) == 0;
+}
+
+bool loadBinary(const std::string& path, CSRGraph& g) {
+    uint32_t n = 0, m = 0, hasXY = 0;
+    if (pf_csr_peek(path.c_str(), &n, &m, &hasXY) != 0) return false;
+    std::vector<uint32_t> row((size_t)n + 1), col(m), w(m);
+    EdgeList el;
+    el.n = n;
+    el.name = "bin";
+    if (hasXY) {
+        el.x.resize(n);
+        el.y.resize(n);
+    }
+    if (pf_csr_load(path.c_str(), row.data(), col.data(), w.data(), hasXY ? el.x.data() : nullptr,
+                    hasXY ? el.y.data() : nullptr) != 0)
+        return false;
+    el.src.resize(m);
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint32_t k = row[u]; k < row[u + 1]; ++k) el.src[k] = u;
+    el.dst = std::move(col);
+    el.w = std::move(w);
+    g = buildCSR(std::move(el));   // re-derives statistics and hScale, as the reference does
+    return true;
+}
+
+std::vector<u32> pickSources(const CSRGraph& g, u32 count, u64 seed) {
+    std::vector<u32> s;
+    const uint64_t base = streamBase(seed, 4);
+    for (uint64_t j = 0; s.size() < count && j < (uint64_t)count * 1000u + 1000u; ++j) {
+        const u32 v = (u32)(draw(base, j) % g.n);
+        if (g.degree(v) > 0) s.push_back(v);
+    }
+    while (s.size() < count) s.push_back(0);
+    return s;
+}

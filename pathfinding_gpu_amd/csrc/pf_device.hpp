@@ -1,0 +1,978 @@
+// pf_device.hpp -- device code of the gfx950 SSSP / A* engine.
+//
+// One kernel template, stepKernel<ASTAR, AM, MAP>, executes one "step" of a
+// device-resident state machine: a light or heavy relaxation pass, a far-pile
+// minimum, a far-pile partition, a frontier pass, a naive sweep or one of the
+// three prefix-scan compaction passes.  The phase and every queue size live
+// in a control block (Ctl) in HBM, so the host never has to read a counter
+// between steps.  The last workgroup to finish a step (atomic ticket) runs
+// the controller that picks the next phase.  The same kernel runs
+//   * one step per launch (host-driven loop, as the reference does),
+//   * S steps per replay of a captured hipGraph,
+//   * every step in one cooperative launch (persistent, grid barrier).
+//
+// Reference counterparts: src/gpu_kernels.cuh (relaxFrontierKernel :207-288,
+// naiveRelaxKernel :297-329, initStateKernel :334-337, seedKernel :340-352,
+// heuristicKernel :355-358, lightEndKernel :361-371, scatterKernel :376-383,
+// farMinKernel :386-410, farPartitionKernel :413-436) and the host loops of
+// src/gpu_sssp.cu:257-456 whose decisions the controller makes on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+using u8 = uint8_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr u32 INF = 0xFFFFFFFFu;       // INF_DIST (include/common.h:21)
+constexpr u32 NOPAR = 0xFFFFFFFFu;     // NO_PARENT (include/common.h:22)
+constexpr u64 STATE_INIT = ~0ull;      // (INF<<32)|NO_PARENT (src/gpu_kernels.cuh:36)
+constexpr u64 NOKEY = ~0ull;
+
+// Atomic modes / mappings: values of pf_atomic / pf_mapping.
+enum { AM_MIN_RBA = 0, AM_CAS_RBA = 1, AM_MIN_NOCHECK = 2 };
+enum { MAP_THREAD = 0, MAP_WAVE = 1, MAP_HYBRID = 2 };
+enum { ALGO_NAIVE = 0, ALGO_FRONTIER = 1, ALGO_DELTA = 2 };
+enum { HK_NONE = 0, HK_SCALED = 1, HK_FLOOR_EUCLID = 2 };
+
+enum Phase : u32 {
+    PH_LIGHT = 0,        // relax light edges of the bucket, mark R
+    PH_HEAVY = 1,        // relax heavy edges of R
+    PH_FARMIN = 2,       // per-slot minimum valid key of the far pile
+    PH_PART = 3,         // split the far pile around the new bounds
+    PH_FRONTIER = 4,     // Bellman-Ford-Moore frontier pass (all edges)
+    PH_NAIVE = 5,        // all-vertex sweep with active flags
+    PH_SCAN_RELAX = 6,   // frontier pass that only sets flags
+    PH_SCAN_TILES = 7,   // per-tile flag sums
+    PH_SCAN_SCATTER = 8, // tile-local scan + scatter into the next frontier
+    PH_DONE = 9
+};
+
+enum Counter : u32 { C_NEAR0 = 0, C_NEAR1 = 1, C_FAR0 = 2, C_FAR1 = 3, C_R = 4, C_CHANGED = 5, C_COUNT = 8 };
+
+enum CtlError : u32 { E_NONE = 0, E_NOT_CONVERGED = 1, E_BARRIER = 2 };
+
+// Control block (HBM).  Fields are read by every workgroup at the start of a
+// step and written only by the controller (the step's last workgroup).
+struct Ctl {
+    u32 phase, in, fcur, cur;   // phase, near input buffer, current far buffer, naive flag buffer
+    u32 cnt[C_COUNT];           // queue sizes, naive "changed"
+    u32 arrive, gen;            // step ticket / barrier generation
+    u32 steps, iters, buckets, error;
+    u32 scanTotal, epochEnd, pause, maxSteps;
+    u32 pad[8];
+};
+
+struct DevGraph {
+    const u32* row;       // row_offsets [n+1]
+    const uint2* adj;     // {dst, w} [m], each row sorted by (w, dst)
+    const u32* lightEnd;  // [n] first heavy arc of each row (for the run's delta)
+    const int2* xyi;      // integer coordinates [n] (FLOOR_EUCLID) or null
+    const double2* xyd;   // double coordinates [n] (SCALED) or null
+    u32 n, m;
+};
+
+// Everything a step needs.  Passed by value; identical for every step of a run.
+struct Run {
+    DevGraph g;
+    u64* state;                  // [K*n] (dist<<32 | parent)
+    Ctl* ctl;
+    unsigned long long* stats;   // [5] edges, atomics, updates, pushes, -
+    u32* nearQ[2];
+    u32* nearMask[2];
+    u32* farQ[2];
+    u32* farMask;
+    u32* rQ;
+    u32* rMask;
+    u32* flags;                  // prefix-scan mode [K*n]
+    u32* tileSums;               // prefix-scan mode [numTiles]
+    u8* act[2];                  // naive [n]
+    u64* bound;                  // [K] upper bound of each slot's current bucket
+    u64* newBound;               // [K]
+    unsigned long long* minKey;  // [K]
+    const u32* target;           // [K] A* targets
+    u32* slotDone;               // [K] set when a slot has no work left
+    double hScale;
+    double invN;                 // 1/n for the slot index of k*n+v
+    u32 K;                       // slots in this run
+    u32 delta;
+    u32 hkind;
+    u32 waveCutoff, blockCutoff;
+    u32 numTiles, tileItems;     // prefix-scan tiling
+    u32 total;                   // K*n
+};
+
+struct LocalStats {
+    u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u64 pack(u32 d, u32 p) { return ((u64)d << 32) | (u64)p; }
+__device__ __forceinline__ u32 distOf(u64 s) { return (u32)(s >> 32); }
+
+// Loads that must see other workgroups' writes inside one kernel (control
+// block, barrier generation): agent-scope atomic loads.
+__device__ __forceinline__ u32 ldAgent(const u32* p) {
+    return __hip_atomic_load(const_cast<u32*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stAgent(u32* p, u32 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u32 laneId() { return __lane_id(); }
+
+// slot k of element k*n+v
+__device__ __forceinline__ u32 slotOf(const Run& r, u32 gi) {
+    if (r.K == 1) return 0;
+    u32 k = (u32)((double)gi * r.invN);
+    if ((u64)k * r.g.n > gi) --k;
+    else if ((u64)(k + 1) * r.g.n <= gi) ++k;
+    return k;
+}
+
+// Wave-aggregated increment: one atomicAdd per group of active lanes
+// (replaces warpAggregatedInc, src/gpu_kernels.cuh:135-141, for 64 lanes).
+__device__ __forceinline__ u32 waveAggInc(u32* ctr) {
+    const u64 active = __builtin_amdgcn_read_exec();
+    const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(active >> 32), __builtin_amdgcn_mbcnt_lo((u32)active, 0u));
+    u32 base = 0;
+    if (rank == 0) base = atomicAdd(ctr, (u32)__popcll(active));
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + rank;
+}
+__device__ __forceinline__ void pushQ(u32* data, u32* ctr, u32 item) { data[waveAggInc(ctr)] = item; }
+
+// Read-then-atomicOr dedup bit (src/gpu_kernels.cuh:124-129).
+__device__ __forceinline__ bool testAndSetBit(u32* mask, u32 i) {
+    u32* wp = mask + (i >> 5);
+    const u32 bit = 1u << (i & 31u);
+    if (*wp & bit) return false;
+    return (atomicOr(wp, bit) & bit) == 0u;
+}
+__device__ __forceinline__ void clearBit(u32* mask, u32 i) { atomicAnd(mask + (i >> 5), ~(1u << (i & 31u))); }
+
+// Exact floor(sqrt(a)) for a < 2^53.
+__device__ __forceinline__ u32 isqrtFloor(u64 a) {
+    u64 s = (u64)__dsqrt_rn((double)a);
+    while (s * s > a) --s;
+    while ((s + 1) * (s + 1) <= a) ++s;
+    return (u32)s;
+}
+
+// Heuristic of vertex v towards a target with coordinates (tx, ty).
+// HK_SCALED repeats include/heuristic.h:19-25 operation by operation
+// (correctly rounded sqrt, no contraction) so values equal the host's.
+struct Target {
+    int2 ti;
+    double2 td;
+};
+__device__ __forceinline__ Target loadTarget(const Run& r, u32 t) {
+    Target x;
+    x.ti = r.g.xyi ? r.g.xyi[t] : make_int2(0, 0);
+    x.td = r.g.xyd ? r.g.xyd[t] : make_double2(0.0, 0.0);
+    return x;
+}
+__device__ __forceinline__ u32 heurScaled(double2 p, double2 t, double scale) {
+#pragma clang fp contract(off)
+    const double dx = p.x - t.x, dy = p.y - t.y;
+    const double s = dx * dx + dy * dy;
+    const double e = __dsqrt_rn(s) * scale * (1.0 - 1e-9);
+    if (!(e > 0.0)) return 0u;
+    if (e >= 4.0e9) return 4000000000u;
+    return (u32)e;
+}
+__device__ __forceinline__ u32 heur(const Run& r, const Target& t, u32 v) {
+    if (r.hkind == HK_FLOOR_EUCLID) {
+        const int2 p = r.g.xyi[v];
+        const long long dx = (long long)p.x - t.ti.x, dy = (long long)p.y - t.ti.y;
+        return isqrtFloor((u64)(dx * dx) + (u64)(dy * dy));
+    }
+    if (r.hkind == HK_SCALED) return heurScaled(r.g.xyd[v], t.td, r.hScale);
+    return 0u;
+}
+
+// ---------------------------------------------------------------------------
+// relaxation (src/gpu_kernels.cuh:96-121 relaxAtomic, :168-193 relaxRange)
+// ---------------------------------------------------------------------------
+template <int AM>
+__device__ __forceinline__ bool relaxAtomic(u64* addr, u32 nd, u32 parent, LocalStats& ls) {
+    const u64 nv = pack(nd, parent);
+    if constexpr (AM == AM_MIN_NOCHECK) {
+        ls.atomics++;
+        const u64 old = atomicMin((unsigned long long*)addr, (unsigned long long)nv);
+        return distOf(old) > nd;
+    } else {
+        u64 cur = *addr;                       // read-before-atomic; a stale value is >= the true one
+        if (cur <= nv) return false;
+        if constexpr (AM == AM_MIN_RBA) {
+            ls.atomics++;
+            const u64 old = atomicMin((unsigned long long*)addr, (unsigned long long)nv);
+            return distOf(old) > nd;
+        } else {
+            while (true) {
+                ls.atomics++;
+                const u64 old = atomicCAS((unsigned long long*)addr, (unsigned long long)cur, (unsigned long long)nv);
+                if (old == cur) return distOf(old) > nd;
+                cur = old;
+                if (cur <= nv) return false;
+            }
+        }
+    }
+}
+
+// Outputs of a relaxation pass.
+struct Outs {
+    u32* nearQ;
+    u32* nearCnt;
+    u32* nearMask;
+    u32* farQ;
+    u32* farCnt;
+    u32* farMask;
+    u32* flags;   // prefix-scan mode: set flags[gv] instead of pushing
+};
+
+// Per-vertex context of an expanded frontier element.
+struct Item {
+    u32 u, du, k, beg, end;
+    u64 ub;       // A*: dist[target] of the slot (prune key > ub)
+    u64 bnd;      // bucket bound of the slot (key < bnd -> near)
+    Target t;
+};
+
+template <bool ASTAR, int AM>
+__device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a, const Outs& o, LocalStats& ls) {
+    const u32 v = a.x;
+    const u64 nd = (u64)it.du + a.y;
+    ls.edges++;
+    if (nd >= INF) return;
+    u64 key = nd;
+    if constexpr (ASTAR) {
+        key += heur(r, it.t, v);
+        if (key > it.ub) return;               // strict: ties with dist[target] stay explored
+    }
+    const u32 gv = it.k * r.g.n + v;
+    if (relaxAtomic<AM>(&r.state[gv], (u32)nd, it.u, ls)) {
+        ls.updates++;
+        if (o.flags) {
+            o.flags[gv] = 1u;
+            ls.pushes++;
+        } else if (key < it.bnd) {
+            if (testAndSetBit(o.nearMask, gv)) { pushQ(o.nearQ, o.nearCnt, gv); ls.pushes++; }
+        } else {
+            if (testAndSetBit(o.farMask, gv)) { pushQ(o.farQ, o.farCnt, gv); ls.pushes++; }
+        }
+    }
+}
+
+template <bool ASTAR, int AM>
+__device__ __forceinline__ void relaxRange(const Run& r, const Item& it, u32 beg, u32 end, u32 stride,
+                                           const Outs& o, LocalStats& ls) {
+    for (u32 e = beg; e < end; e += stride) relaxEdge<ASTAR, AM>(r, it, r.g.adj[e], o, ls);
+}
+
+// Wave-wide inclusive sum (64 lanes).
+__device__ __forceinline__ u32 waveInclusiveSum(u32 x) {
+    const u32 lane = laneId();
+#pragma unroll
+    for (u32 off = 1; off < 64; off <<= 1) {
+        const u32 y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Per-wave staging area of the gather tier (dynamic LDS, 64 entries per wave).
+struct GatherSlot {
+    u32 excl, u, du, k, beg;
+    u32 ubLo, ubHi, bLo, bHi;
+    u32 pad;
+};
+static_assert(sizeof(GatherSlot) == 40, "gather slot");
+
+// ---------------------------------------------------------------------------
+// Frontier expansion with thread / wave / hybrid mapping.
+//   hybrid = workgroup tier (deg >= blockCutoff, owner elected through LDS,
+//            as the reference's CTA tier), wave tier (deg >= waveCutoff,
+//            64-bit ballot), and a load-balanced gather for the rest: each
+//            wave scans its lanes' degrees, stages the vertices in LDS and
+//            hands out consecutive arcs to consecutive lanes.
+// ---------------------------------------------------------------------------
+template <bool ASTAR, int AM, int MAP>
+__device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMask, int sel, bool markR,
+                               bool useBound, const Outs& o, LocalStats& ls, u32* s_owner, Item* s_item,
+                               GatherSlot* gather) {
+    const u32 bs = blockDim.x;
+    const u32 n = r.g.n;
+    for (u32 base0 = blockIdx.x * bs; base0 < size; base0 += gridDim.x * bs) {   // block-uniform
+        const u32 idx = base0 + threadIdx.x;
+        Item it;
+        it.u = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
+        it.ub = NOKEY; it.bnd = NOKEY;
+        if (idx < size) {
+            const u32 gi = in[idx];
+            if (inMask) clearBit(inMask, gi);          // may re-enter the frontier later
+            it.k = slotOf(r, gi);
+            it.u = gi - it.k * n;
+            it.du = distOf(r.state[gi]);
+            bool active = it.du != INF;
+            if constexpr (ASTAR) {
+                const u32 t = r.target[it.k];
+                it.t = loadTarget(r, t);
+                it.ub = distOf(r.state[it.k * n + t]);
+                if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
+            }
+            if (active) {
+                if (markR && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &r.ctl->cnt[C_R], gi);
+                const u32 r0 = r.g.row[it.u], r1 = r.g.row[it.u + 1];
+                if (sel == 0) { it.beg = r0; it.end = r1; }
+                else if (sel == 1) { it.beg = r0; it.end = r.g.lightEnd[it.u]; }
+                else { it.beg = r.g.lightEnd[it.u]; it.end = r1; }
+                it.bnd = useBound ? r.bound[it.k] : NOKEY;
+            }
+        }
+        bool done = it.end <= it.beg;
+
+        if constexpr (MAP == MAP_HYBRID) {           // workgroup tier
+            while (true) {
+                if (threadIdx.x == 0) *s_owner = INF;
+                __syncthreads();
+                if (!done && it.end - it.beg >= r.blockCutoff) *s_owner = threadIdx.x;   // benign race
+                __syncthreads();
+                const u32 owner = *s_owner;
+                if (owner == INF) break;
+                if (owner == threadIdx.x) { *s_item = it; done = true; }
+                __syncthreads();
+                const Item bi = *s_item;
+                relaxRange<ASTAR, AM>(r, bi, bi.beg + threadIdx.x, bi.end, bs, o, ls);
+                __syncthreads();
+            }
+        }
+
+        if constexpr (MAP == MAP_HYBRID || MAP == MAP_WAVE) {   // wave tier
+            const u32 lane = laneId();
+            const bool want = !done && (MAP == MAP_WAVE || it.end - it.beg >= r.waveCutoff);
+            u64 ballot = __ballot(want);
+            while (ballot) {
+                const int leader = __ffsll((unsigned long long)ballot) - 1;
+                ballot &= ballot - 1;
+                Item wi;
+                wi.u = __shfl(it.u, leader, 64);
+                wi.du = __shfl(it.du, leader, 64);
+                wi.k = __shfl(it.k, leader, 64);
+                wi.beg = __shfl(it.beg, leader, 64);
+                wi.end = __shfl(it.end, leader, 64);
+                wi.ub = ((u64)__shfl((u32)(it.ub >> 32), leader, 64) << 32) | __shfl((u32)it.ub, leader, 64);
+                wi.bnd = ((u64)__shfl((u32)(it.bnd >> 32), leader, 64) << 32) | __shfl((u32)it.bnd, leader, 64);
+                if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
+                relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
+            }
+            if (want) done = true;
+        }
+
+        if constexpr (MAP == MAP_THREAD) {
+            if (!done) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
+        } else if constexpr (MAP == MAP_HYBRID) {    // gather tier
+            const u32 lane = laneId();
+            const u32 deg = done ? 0u : it.end - it.beg;
+            const u32 incl = waveInclusiveSum(deg);
+            const u32 total = __shfl(incl, 63, 64);
+            GatherSlot* gs = gather + (threadIdx.x >> 6) * 64;
+            GatherSlot me;
+            me.excl = incl - deg; me.u = it.u; me.du = it.du; me.k = it.k; me.beg = it.beg;
+            me.ubLo = (u32)it.ub; me.ubHi = (u32)(it.ub >> 32);
+            me.bLo = (u32)it.bnd; me.bHi = (u32)(it.bnd >> 32);
+            me.pad = 0;
+            gs[lane] = me;
+            __syncthreads();
+            {
+                for (u32 e0 = 0; e0 < total; e0 += 64) {
+                    const u32 e = e0 + lane;
+                    if (e < total) {
+                        u32 lo = 0, hi = 63;                 // last lane with excl <= e
+                        while (lo < hi) {
+                            const u32 mid = (lo + hi + 1) >> 1;
+                            if (gs[mid].excl <= e) lo = mid; else hi = mid - 1;
+                        }
+                        const GatherSlot ow = gs[lo];
+                        Item gi;
+                        gi.u = ow.u; gi.du = ow.du; gi.k = ow.k;
+                        gi.ub = ((u64)ow.ubHi << 32) | ow.ubLo;
+                        gi.bnd = ((u64)ow.bHi << 32) | ow.bLo;
+                        if constexpr (ASTAR) gi.t = loadTarget(r, r.target[gi.k]);
+                        relaxEdge<ASTAR, AM>(r, gi, r.g.adj[ow.beg + (e - ow.excl)], o, ls);
+                    }
+                }
+            }
+            __syncthreads();   // the staging area is rewritten by the next chunk
+        }
+    }
+}
+
+// Naive synchronous relaxation, one thread per vertex (src/gpu_kernels.cuh:297-329).
+template <bool ASTAR, int AM>
+__device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, LocalStats& ls) {
+    const u32 n = r.g.n;
+    Target t;
+    u64 ubState = NOKEY;
+    (void)ubState;
+    if constexpr (ASTAR) t = loadTarget(r, r.target[0]);
+    for (u32 u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        if (!actIn[u]) continue;
+        actIn[u] = 0;
+        Item it;
+        it.u = u; it.k = 0; it.du = distOf(r.state[u]);
+        it.bnd = NOKEY; it.ub = NOKEY; it.t = t;
+        bool active = it.du != INF;
+        if constexpr (ASTAR) {
+            it.ub = distOf(r.state[r.target[0]]);
+            if (active && (u64)it.du + heur(r, t, u) > it.ub) active = false;
+        }
+        if (!active) continue;
+        const u32 r1 = r.g.row[u + 1];
+        for (u32 e = r.g.row[u]; e < r1; ++e) {
+            const uint2 a = r.g.adj[e];
+            const u64 nd = (u64)it.du + a.y;
+            ls.edges++;
+            if (nd >= INF) continue;
+            if constexpr (ASTAR) {
+                if (nd + heur(r, t, a.x) > it.ub) continue;
+            }
+            if (relaxAtomic<AM>(&r.state[a.x], (u32)nd, u, ls)) {
+                ls.updates++;
+                actOut[a.x] = 1;
+                if (ldAgent(changed) == 0u) stAgent(changed, 1u);
+            }
+        }
+    }
+}
+
+// Far pile minimum (src/gpu_kernels.cuh:386-410), per slot.
+template <bool ASTAR>
+__device__ void farMin(const Run& r, const u32* far, u32 size) {
+    const u32 n = r.g.n;
+    for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx - threadIdx.x < size; idx += gridDim.x * blockDim.x) {
+        unsigned long long key = NOKEY;
+        u32 k = 0;
+        if (idx < size) {
+            const u32 gi = far[idx];
+            k = slotOf(r, gi);
+            const u32 d = distOf(r.state[gi]);
+            if (d != INF) {
+                u64 kk = d;
+                bool ok = true;
+                if constexpr (ASTAR) {
+                    const u32 t = r.target[k];
+                    kk += heur(r, loadTarget(r, t), gi - k * n);
+                    ok = kk <= (u64)distOf(r.state[k * n + t]);
+                }
+                if (ok && kk >= r.bound[k]) key = kk;   // below the old bound: already settled in that bucket
+            }
+        }
+        if (r.K == 1) {                              // one slot: wave minimum, one atomic per wave
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o2 = __shfl_down(key, off, 64);
+                key = o2 < key ? o2 : key;
+            }
+            if (laneId() == 0 && key != NOKEY) atomicMin(&r.minKey[0], key);
+        } else if (key != NOKEY && key < r.minKey[k]) {
+            atomicMin(&r.minKey[k], key);
+        }
+    }
+}
+
+// Far pile partition (src/gpu_kernels.cuh:413-436).
+template <bool ASTAR>
+__device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ, u32* nearCnt, u32* nearMask,
+                             u32* farOut, u32* farOutCnt) {
+    const u32 n = r.g.n;
+    for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size; idx += gridDim.x * blockDim.x) {
+        const u32 gi = far[idx];
+        const u32 k = slotOf(r, gi);
+        const u32 d = distOf(r.state[gi]);
+        u64 key = d;
+        bool ok = d != INF;
+        if constexpr (ASTAR) {
+            const u32 t = r.target[k];
+            key += heur(r, loadTarget(r, t), gi - k * n);
+            ok = ok && key <= (u64)distOf(r.state[k * n + t]);
+        }
+        ok = ok && key >= r.bound[k];
+        if (!ok) {
+            clearBit(r.farMask, gi);                 // stale or pruned
+        } else if (key < r.newBound[k]) {
+            clearBit(r.farMask, gi);                 // leaves the far pile for the new bucket
+            if (testAndSetBit(nearMask, gi)) pushQ(nearQ, nearCnt, gi);
+        } else {
+            pushQ(farOut, farOutCnt, gi);            // stays (bit unchanged)
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// prefix-scan compaction (cub::DeviceScan::ExclusiveSum + scatterKernel,
+// src/gpu_sssp.cu:312-313, src/gpu_kernels.cuh:376-383), on 64-lane waves
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ u32 blockExclusiveScan(u32 x, u32* waveTotals, u32& excl) {
+    const u32 lane = laneId(), wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const u32 incl = waveInclusiveSum(x);
+    if (lane == 63) waveTotals[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const u32 t = lane < nw ? waveTotals[lane] : 0u;
+        const u32 ti = waveInclusiveSum(t);
+        if (lane < nw) waveTotals[lane] = ti - t;
+        if (lane == 63) waveTotals[16] = ti;
+    }
+    __syncthreads();
+    excl = waveTotals[wave] + incl - x;
+    const u32 tot = waveTotals[16];
+    __syncthreads();
+    return tot;
+}
+
+__device__ void scanTiles(const Run& r, u32* waveTotals) {
+    const u32 per = r.tileItems / blockDim.x;
+    for (u32 t = blockIdx.x; t < r.numTiles; t += gridDim.x) {
+        const u32 b = t * r.tileItems;
+        u32 s = 0;
+        for (u32 j = 0; j < per; ++j) {
+            const u32 i = b + j * blockDim.x + threadIdx.x;
+            if (i < r.total) s += r.flags[i];
+        }
+        u32 ex;
+        const u32 tot = blockExclusiveScan(s, waveTotals, ex);
+        if (threadIdx.x == 0) r.tileSums[t] = tot;
+    }
+}
+
+__device__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
+    const u32 per = r.tileItems / blockDim.x;
+    for (u32 t = blockIdx.x; t < r.numTiles; t += gridDim.x) {
+        const u32 b = t * r.tileItems;
+        u32 base = r.tileSums[t];
+        // items are visited in ascending order: chunk j covers [b + j*bs, b + (j+1)*bs)
+        for (u32 j = 0; j < per; ++j) {
+            const u32 i = b + j * blockDim.x + threadIdx.x;
+            const u32 f = i < r.total ? r.flags[i] : 0u;
+            u32 ex;
+            const u32 tot = blockExclusiveScan(f, waveTotals, ex);
+            if (f) {
+                outQ[base + ex] = i;
+                r.flags[i] = 0u;
+            }
+            base += tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// controller: the host loop decisions of src/gpu_sssp.cu:257-456, on device
+// ---------------------------------------------------------------------------
+struct CtlView {
+    u32 phase, in, fcur, cur;
+};
+
+__device__ void controller(const Run& r, u32* waveTotals) {
+    Ctl* c = r.ctl;
+    __shared__ u32 s_next, s_any, s_resetMin, s_setBounds, s_scan;
+    if (threadIdx.x == 0) {
+        u32 phase = c->phase;
+        u32 in = c->in, fcur = c->fcur;
+        u32 next = PH_DONE;
+        u32 resetMin = 0, setBounds = 0, scan = 0;
+        auto bucketDone = [&]() {
+            c->buckets++;
+            if (c->cnt[C_FAR0 + fcur] == 0) next = PH_DONE;
+            else { next = PH_FARMIN; resetMin = 1; }
+        };
+        switch (phase) {
+            case PH_LIGHT: {
+                if (c->cnt[C_NEAR0 + in]) c->iters++;
+                in ^= 1u;
+                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else if (c->cnt[C_R]) { next = PH_HEAVY; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else { c->cnt[C_NEAR0 + (in ^ 1u)] = 0; bucketDone(); }
+                break;
+            }
+            case PH_HEAVY: {
+                c->iters++;
+                c->cnt[C_R] = 0;
+                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else bucketDone();
+                break;
+            }
+            case PH_FARMIN: next = PH_PART; setBounds = 1; break;   // decided below over all slots
+            case PH_PART: {
+                c->cnt[C_FAR0 + fcur] = 0;
+                fcur ^= 1u;
+                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else if (c->cnt[C_FAR0 + fcur]) { next = PH_FARMIN; resetMin = 1; }
+                else next = PH_DONE;
+                break;
+            }
+            case PH_FRONTIER: {
+                if (c->cnt[C_NEAR0 + in]) c->iters++;
+                in ^= 1u;
+                if (c->cnt[C_NEAR0 + in]) { next = PH_FRONTIER; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else next = PH_DONE;
+                break;
+            }
+            case PH_NAIVE: {
+                c->iters++;
+                c->cur ^= 1u;
+                if (c->cnt[C_CHANGED]) { next = PH_NAIVE; c->cnt[C_CHANGED] = 0; }
+                else next = PH_DONE;
+                break;
+            }
+            case PH_SCAN_RELAX: {
+                if (c->cnt[C_NEAR0 + in]) c->iters++;
+                next = PH_SCAN_TILES;
+                break;
+            }
+            case PH_SCAN_TILES: next = PH_SCAN_SCATTER; scan = 1; break;
+            case PH_SCAN_SCATTER: {
+                in ^= 1u;
+                c->cnt[C_NEAR0 + in] = c->scanTotal;
+                c->cnt[C_NEAR0 + (in ^ 1u)] = 0;
+                next = c->scanTotal ? PH_SCAN_RELAX : PH_DONE;
+                break;
+            }
+            default: next = PH_DONE;
+        }
+        c->in = in;
+        c->fcur = fcur;
+        s_next = next;
+        s_resetMin = resetMin;
+        s_setBounds = setBounds;
+        s_scan = scan;
+    }
+    __syncthreads();
+    if (s_resetMin) {
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.minKey[k] = NOKEY;
+    }
+    if (s_setBounds) {
+        bool any = false;
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) {
+            const unsigned long long mk = r.minKey[k];
+            if (mk != NOKEY) {
+                r.newBound[k] = (mk / r.delta + 1) * (u64)r.delta;   // src/gpu_sssp.cu:441
+                any = true;
+            } else {
+                r.newBound[k] = r.bound[k];
+                if (!r.slotDone[k]) r.slotDone[k] = 1u;
+            }
+        }
+        const int anyAll = __syncthreads_or(any);
+        if (threadIdx.x == 0) {
+            if (anyAll) c->cnt[C_FAR0 + (c->fcur ^ 1u)] = 0;
+            else s_next = PH_DONE;
+        }
+    }
+    if (s_scan) {   // exclusive scan of tile sums in place
+        const u32 nt = r.numTiles;
+        const u32 per = (nt + blockDim.x - 1) / blockDim.x;
+        const u32 b = threadIdx.x * per;
+        u32 s = 0;
+        for (u32 j = 0; j < per; ++j)
+            if (b + j < nt) s += r.tileSums[b + j];
+        u32 ex;
+        const u32 tot = blockExclusiveScan(s, waveTotals, ex);
+        for (u32 j = 0; j < per; ++j)
+            if (b + j < nt) { const u32 v = r.tileSums[b + j]; r.tileSums[b + j] = ex; ex += v; }
+        if (threadIdx.x == 0) c->scanTotal = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // bounds move forward only once the partition has used old and new
+        if (c->phase == PH_PART) { /* handled below */ }
+        u32 next = s_next;
+        c->steps++;
+        if (c->steps >= c->maxSteps && next != PH_DONE) { c->error = E_NOT_CONVERGED; next = PH_DONE; }
+        if (c->epochEnd && c->steps >= c->epochEnd && next != PH_DONE) c->pause = 1u;
+        s_next = next;
+    }
+    __syncthreads();
+    if (c->phase == PH_PART) {
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.bound[k] = r.newBound[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) c->phase = s_next;
+}
+
+// ---------------------------------------------------------------------------
+// the step kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void flushStats(unsigned long long* stats, const LocalStats& ls) {
+    unsigned long long a = ls.edges, b = ls.atomics, c = ls.updates, d = ls.pushes;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+        d += __shfl_down(d, off, 64);
+    }
+    if (laneId() == 0) {
+        if (a) atomicAdd(&stats[0], a);
+        if (b) atomicAdd(&stats[1], b);
+        if (c) atomicAdd(&stats[2], c);
+        if (d) atomicAdd(&stats[3], d);
+    }
+}
+
+constexpr u32 SPIN_LIMIT = 1u << 25;   // bounded barrier wait (about a few seconds)
+
+template <bool ASTAR, int AM, int MAP>
+__global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
+    extern __shared__ GatherSlot gather[];
+    __shared__ u32 s_phase, s_in, s_fcur, s_cur, s_size, s_last, s_gen, s_exit;
+    __shared__ u32 s_owner;
+    __shared__ Item s_item;
+    __shared__ u32 waveTotals[17];
+    LocalStats ls;
+    Ctl* c = r.ctl;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const u32 ph = ldAgent(&c->phase);
+            const u32 in = ldAgent(&c->in), fc = ldAgent(&c->fcur);
+            s_phase = ph; s_in = in; s_fcur = fc; s_cur = ldAgent(&c->cur);
+            u32 sz = 0;
+            switch (ph) {
+                case PH_LIGHT: case PH_FRONTIER: case PH_SCAN_RELAX: sz = ldAgent(&c->cnt[C_NEAR0 + in]); break;
+                case PH_HEAVY: sz = ldAgent(&c->cnt[C_R]); break;
+                case PH_FARMIN: case PH_PART: sz = ldAgent(&c->cnt[C_FAR0 + fc]); break;
+                default: sz = 0;
+            }
+            s_size = sz;
+        }
+        __syncthreads();
+        const u32 phase = s_phase, in = s_in, fcur = s_fcur, size = s_size;
+        if (phase == PH_DONE) break;
+        switch (phase) {
+            case PH_LIGHT: {
+                const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
+                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr};
+                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 1, true, true, o, ls,
+                                               &s_owner, &s_item, gather);
+                break;
+            }
+            case PH_HEAVY: {
+                const Outs o{r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
+                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr};
+                expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, &s_owner, &s_item,
+                                               gather);
+                break;
+            }
+            case PH_FRONTIER: {
+                const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
+                             r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u], nullptr};
+                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 0, false, false, o, ls,
+                                               &s_owner, &s_item, gather);
+                break;
+            }
+            case PH_SCAN_RELAX: {
+                const Outs o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, r.flags};
+                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, nullptr, 0, false, false, o, ls, &s_owner,
+                                               &s_item, gather);
+                break;
+            }
+            case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
+            case PH_SCAN_SCATTER: scanScatter(r, r.nearQ[in ^ 1u], waveTotals); break;
+            case PH_FARMIN: farMin<ASTAR>(r, r.farQ[fcur], size); break;
+            case PH_PART:
+                farPartition<ASTAR>(r, r.farQ[fcur], size, r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
+                                    r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)]);
+                break;
+            case PH_NAIVE:
+                naiveSweep<ASTAR, AM>(r, r.act[s_cur], r.act[s_cur ^ 1u], &c->cnt[C_CHANGED], ls);
+                break;
+            default: break;
+        }
+        // ---- arrive; the last workgroup runs the controller ----
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // every wave's stores, before the ticket
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_gen = ldAgent(&c->gen);
+            const u32 t = atomicAdd(&c->arrive, 1u);
+            s_last = (t == gridDim.x - 1) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (s_last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            controller(r, waveTotals);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                stAgent(&c->arrive, 0u);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                stAgent(&c->gen, s_gen + 1u);
+            }
+        } else if (persistent) {
+            if (threadIdx.x == 0) {
+                u32 spins = 0;
+                while (ldAgent(&c->gen) == s_gen) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > SPIN_LIMIT) { atomicOr(&c->error, (u32)E_BARRIER); break; }
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        if (!persistent) break;
+        if (threadIdx.x == 0) s_exit = ldAgent(&c->pause) | (ldAgent(&c->error) & E_BARRIER);
+        __syncthreads();
+        if (s_exit) break;
+    }
+    flushStats(r.stats, ls);
+}
+
+// ---------------------------------------------------------------------------
+// run set-up kernels
+// ---------------------------------------------------------------------------
+// initStateKernel (src/gpu_kernels.cuh:334-337) + mask / flag clearing.
+__global__ void initKernel(u64* state, u64 total, u32* m0, u32* m1, u32* m2, u32* m3, u64 words, u32* flags,
+                           u64 flagCount, u8* a0, u8* a1, u32 actCount) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (u64 i = tid; i < total; i += stride) state[i] = STATE_INIT;
+    for (u64 i = tid; i < words; i += stride) { m0[i] = 0u; m1[i] = 0u; m2[i] = 0u; m3[i] = 0u; }
+    if (flags)
+        for (u64 i = tid; i < flagCount; i += stride) flags[i] = 0u;
+    if (a0)
+        for (u64 i = tid; i < actCount; i += stride) { a0[i] = 0; a1[i] = 0; }
+}
+
+// seedKernel (src/gpu_kernels.cuh:340-352) + control block and per-slot set-up.
+// seeds are element ids k*n+v.  mode: 0 queue (frontier / delta), 1 naive flags.
+__global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 startPhase, u32 maxSteps,
+                           u32 useBound) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nseeds) {
+        const u32 gi = seeds[i];
+        r.state[gi] = pack(0u, NOPAR);
+        if (mode == 0) {
+            r.nearQ[0][i] = gi;
+            atomicOr(r.nearMask[0] + (gi >> 5), 1u << (gi & 31u));
+        } else {
+            r.act[0][gi] = 1;
+        }
+    }
+    if (i < r.K) {   // slot bounds: first bucket holds key0 = h(src) (src/gpu_sssp.cu:367-370)
+        u64 b = NOKEY;
+        if (useBound) {
+            u64 key0 = 0;
+            if (r.hkind != HK_NONE && r.target && i < nseeds) {
+                const u32 gi = seeds[i];
+                const u32 k = slotOf(r, gi);
+                key0 = heur(r, loadTarget(r, r.target[k]), gi - k * r.g.n);
+            }
+            b = (key0 / r.delta + 1) * (u64)r.delta;
+        }
+        r.bound[i] = b;
+        r.newBound[i] = b;
+        r.minKey[i] = NOKEY;
+        r.slotDone[i] = 0u;
+    }
+    if (i == 0) {
+        Ctl* c = r.ctl;
+        c->phase = startPhase;
+        c->in = 0; c->fcur = 0; c->cur = 0;
+        for (int j = 0; j < C_COUNT; ++j) c->cnt[j] = 0;
+        c->cnt[C_NEAR0] = mode == 0 ? nseeds : 0u;
+        c->arrive = 0; c->gen = 0;
+        c->steps = 0; c->iters = 0; c->buckets = 0; c->error = 0;
+        c->scanTotal = 0; c->epochEnd = 0; c->pause = 0; c->maxSteps = maxSteps;
+        r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
+    }
+}
+
+// lightEndKernel (src/gpu_kernels.cuh:361-371).
+__global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, u32* lightEnd) {
+    const u32 u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) {
+        u32 lo = row[u], hi = row[u + 1];
+        while (lo < hi) {
+            const u32 mid = lo + ((hi - lo) >> 1);
+            if (adj[mid].y <= delta) lo = mid + 1; else hi = mid;
+        }
+        lightEnd[u] = lo;
+    }
+}
+
+// heuristicKernel (src/gpu_kernels.cuh:355-358) -- test hook; the engine
+// evaluates h on the fly per relaxed arc instead of storing h[n] per query.
+__global__ void heuristicKernel(Run r, u32 target, u32* h) {
+    const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < r.g.n) h[v] = heur(r, loadTarget(r, target), v);
+}
+
+// Path of each slot, followed from its target on the device (reconstructPath,
+// src/cpu_sssp.cpp:124-135): one thread per slot.
+__global__ void chaseKernel(const u64* state, u32 n, const u32* srcs, const u32* targets, u32 K, u32* paths,
+                            u32 cap, u32* lens, u32* tdist) {
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const u64* st = state + (u64)k * n;
+    const u32 s = srcs[k], t = targets[k];
+    u32* p = paths + (u64)k * cap;
+    const u32 d = distOf(st[t]);
+    tdist[k] = d;
+    u32 len = 0;
+    if (d != INF) {
+        u32 v = t;
+        bool ok = false;
+        for (u32 steps = 0; steps <= n && len < cap; ++steps) {
+            p[len++] = v;
+            if (v == s) { ok = true; break; }
+            v = (u32)st[v];
+            if (v == NOPAR || v >= n) break;
+        }
+        if (!ok) len = 0;
+        for (u32 i = 0; i < len / 2; ++i) { const u32 x = p[i]; p[i] = p[len - 1 - i]; p[len - 1 - i] = x; }
+    }
+    lens[k] = len;
+}
+
+// Reached vertices and their out-arcs over K slots (TEPS numerator).
+__global__ void reachKernel(const u64* state, const u32* row, u32 n, u64 total, unsigned long long* out) {
+    unsigned long long a = 0, b = 0;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
+        if (distOf(state[i]) != INF) {
+            const u32 v = (u32)(i % n);
+            a += 1;
+            b += row[v + 1] - row[v];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    if (laneId() == 0) {
+        if (a) atomicAdd(&out[0], a);
+        if (b) atomicAdd(&out[1], b);
+    }
+}
+
+// Standalone exclusive scan (test hook for the compaction arithmetic):
+// one block, serial chunks per thread.
+__global__ void exclusiveScanKernel(const u32* in, u32* out, u32 count, u32* total) {
+    __shared__ u32 waveTotals[17];
+    const u32 per = (count + blockDim.x - 1) / blockDim.x;
+    const u32 b = threadIdx.x * per;
+    u32 s = 0;
+    for (u32 j = 0; j < per; ++j)
+        if (b + j < count) s += in[b + j];
+    u32 ex;
+    const u32 tot = blockExclusiveScan(s, waveTotals, ex);
+    for (u32 j = 0; j < per; ++j)
+        if (b + j < count) { out[b + j] = ex; ex += in[b + j]; }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+}  // namespace pf

@@ -1,0 +1,257 @@
+"""Python mirror of the reference's GpuSolver (include/gpu_sssp.h:97-120) over the C-ABI.
+
+Every call runs on the HIP engine (libpfgpu.so); a missing library or device
+raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .graph import CSRGraph
+
+INF_DIST = 0xFFFFFFFF
+NO_PARENT = 0xFFFFFFFF
+
+
+class GpuAlgo(IntEnum):          # include/gpu_sssp.h:11-15
+    NaiveRelax = 0
+    Frontier = 1
+    DeltaStep = 2
+
+
+class Mapping(IntEnum):          # include/gpu_sssp.h:18-22 (Warp = one 64-lane wave)
+    Thread = 0
+    Warp = 1
+    Hybrid = 2
+
+
+class AtomicMode(IntEnum):       # include/gpu_sssp.h:25-29
+    MinRBA = 0
+    CasRBA = 1
+    MinNoCheck = 2
+
+
+class Compaction(IntEnum):       # include/gpu_sssp.h:32-35
+    WarpAtomic = 0
+    PrefixScan = 1
+
+
+class Drive(IntEnum):
+    Host = 0
+    Graph = 1
+    Persistent = 2
+
+
+class Heuristic(IntEnum):
+    None_ = 0
+    Scaled = 1
+    FloorEuclid = 2
+
+
+@dataclass
+class GpuOptions:
+    """GpuOptions (include/gpu_sssp.h:42-52) plus the MI355X knobs."""
+
+    algo: GpuAlgo = GpuAlgo.DeltaStep
+    mapping: Mapping = Mapping.Hybrid
+    atomic: AtomicMode = AtomicMode.MinRBA
+    delta: int = 0
+    block_size: int = 256
+    sync_every: int = 8
+    compaction: Compaction = Compaction.WarpAtomic
+    drive: Drive = Drive.Host
+    heuristic: Heuristic = Heuristic.Scaled
+    wave_cutoff: int = 0
+    block_cutoff: int = 0
+    grid_blocks: int = 0
+    max_steps: int = 0
+
+    def to_c(self) -> N.PfOptions:
+        o = N.PfOptions()
+        N.engine().pf_default_options(C.byref(o))
+        o.algo, o.mapping, o.atomic, o.compaction = int(self.algo), int(self.mapping), int(self.atomic), int(self.compaction)
+        o.delta, o.block_size, o.sync_every, o.drive = self.delta, self.block_size, self.sync_every, int(self.drive)
+        o.wave_cutoff, o.block_cutoff, o.grid_blocks = self.wave_cutoff, self.block_cutoff, self.grid_blocks
+        o.heuristic, o.max_steps = int(self.heuristic), self.max_steps
+        return o
+
+
+@dataclass
+class GpuStats:
+    edges_scanned: int = 0
+    atomics_issued: int = 0
+    updates: int = 0
+    pushes: int = 0
+    iterations: int = 0
+    host_syncs: int = 0
+    launches: int = 0
+    buckets: int = 0
+    delta: int = 0
+    steps: int = 0
+    loop_ms: float = 0.0
+
+    @staticmethod
+    def from_c(s: N.PfStats) -> "GpuStats":
+        return GpuStats(s.edges_scanned, s.atomics_issued, s.updates, s.pushes, s.iterations, s.host_syncs,
+                        s.launches, s.buckets, s.delta, s.steps, s.loop_ms)
+
+
+@dataclass
+class GpuRunResult:
+    """GpuRunResult (include/gpu_sssp.h:67-74)."""
+
+    ms: float = 0.0
+    stats: GpuStats = field(default_factory=GpuStats)
+    dist: Optional[np.ndarray] = None
+    parent: Optional[np.ndarray] = None
+    target_dist: int = INF_DIST
+    path: Optional[np.ndarray] = None
+
+
+def device_info(dev: int = 0) -> dict:
+    d = N.PfDeviceInfo()
+    N.check(N.engine().pf_device_query(dev, C.byref(d)))
+    return {k: (getattr(d, k).decode() if isinstance(getattr(d, k), bytes) else getattr(d, k)) for k, _ in d._fields_}
+
+
+def warmup() -> None:
+    N.check(N.engine().pf_warmup())
+
+
+def _describe(g: CSRGraph) -> N.PfGraph:
+    d = N.PfGraph()
+    d.n, d.m = g.n, g.m
+    d.row_offsets, d.col_indices, d.weights = N.ptr(g.row_offsets), N.ptr(g.col_indices), N.ptr(g.weights)
+    d.x, d.y = N.ptr(g.x, C.c_double), N.ptr(g.y, C.c_double)
+    d.h_scale, d.avg_w, d.avg_deg, d.flags = g.h_scale, g.avg_w, g.avg_deg, 0
+    return d
+
+
+def transfer_study(g: CSRGraph, reps: int = 3) -> dict:
+    """gpuTransferStudy (src/gpu_device.cu:82-160): pageable / pinned / pinned + 3 streams (ms)."""
+    b = C.c_double()
+    ms = (C.c_double * 3)()
+    N.check(N.engine().pf_transfer_study(C.byref(_describe(g)), reps, C.byref(b), ms))
+    return {"bytes": b.value, "pageable_ms": ms[0], "pinned_ms": ms[1], "pinned_streams_ms": ms[2]}
+
+
+class GpuSolver:
+    """Graph resident in HBM + working buffers for up to max_batch slots."""
+
+    def __init__(self, g: CSRGraph, max_batch: int = 1):
+        self.g = g   # kept alive: the engine copied it, the reference keeps a reference
+        self._lib = N.engine()
+        h = C.c_void_p()
+        N.check(self._lib.pf_solver_create(C.byref(_describe(g)), max_batch, C.byref(h)))
+        self._h = h
+        self.n = g.n
+        self.max_batch = max_batch
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pf_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_ms(self) -> float:
+        return self._lib.pf_solver_upload_ms(self._h)
+
+    def device_bytes(self) -> int:
+        return self._lib.pf_solver_device_bytes(self._h)
+
+    def auto_delta(self) -> int:
+        return self._lib.pf_solver_auto_delta(self._h)
+
+    def sssp(self, src: int, opt: GpuOptions = GpuOptions(), copy_back: bool = True) -> GpuRunResult:
+        st = N.PfStats()
+        dist = np.empty(self.n, np.uint32) if copy_back else None
+        par = np.empty(self.n, np.uint32) if copy_back else None
+        N.check(self._lib.pf_sssp(self._h, src, C.byref(opt.to_c()), N.ptr(dist), N.ptr(par), C.byref(st)))
+        return GpuRunResult(ms=st.ms, stats=GpuStats.from_c(st), dist=dist, parent=par)
+
+    def astar(self, src: int, dst: int, opt: GpuOptions = GpuOptions(), path_cap: Optional[int] = None) -> GpuRunResult:
+        r = self.astar_batch([src], [dst], opt, path_cap)
+        return GpuRunResult(ms=r["ms"], stats=r["stats"], target_dist=int(r["dist"][0]), path=r["paths"][0])
+
+    def astar_batch(self, srcs: Sequence[int], dsts: Sequence[int], opt: GpuOptions = GpuOptions(),
+                    path_cap: Optional[int] = None) -> dict:
+        k = len(srcs)
+        s = np.ascontiguousarray(srcs, np.uint32)
+        t = np.ascontiguousarray(dsts, np.uint32)
+        cap = int(path_cap or min(self.n, 1 << 16))
+        paths = np.empty(k * cap, np.uint32)
+        lens = np.empty(k, np.uint32)
+        td = np.empty(k, np.uint32)
+        st = N.PfStats()
+        N.check(self._lib.pf_astar_batch(self._h, N.ptr(s), N.ptr(t), k, C.byref(opt.to_c()), N.ptr(td), N.ptr(paths),
+                                         cap, N.ptr(lens), C.byref(st)))
+        if path_cap is None and any(td[i] != INF_DIST and lens[i] == 0 for i in range(k)) and cap < self.n:
+            return self.astar_batch(srcs, dsts, opt, path_cap=self.n)
+        out = [paths[i * cap: i * cap + int(lens[i])].copy() for i in range(k)]
+        return {"ms": st.ms, "stats": GpuStats.from_c(st), "dist": td, "paths": out}
+
+    def multi_source(self, srcs: Sequence[int], opt: GpuOptions = GpuOptions(), copy_back: bool = True,
+                     parents: bool = False) -> GpuRunResult:
+        k = len(srcs)
+        s = np.ascontiguousarray(srcs, np.uint32)
+        dist = np.empty(k * self.n, np.uint32) if copy_back else None
+        par = np.empty(k * self.n, np.uint32) if (copy_back and parents) else None
+        st = N.PfStats()
+        N.check(self._lib.pf_multi_source(self._h, N.ptr(s), k, C.byref(opt.to_c()), N.ptr(dist), N.ptr(par),
+                                          C.byref(st)))
+        return GpuRunResult(ms=st.ms, stats=GpuStats.from_c(st),
+                            dist=None if dist is None else dist.reshape(k, self.n),
+                            parent=None if par is None else par.reshape(k, self.n))
+
+    def multi_seed(self, seeds: Sequence[int], opt: GpuOptions = GpuOptions(), copy_back: bool = True) -> GpuRunResult:
+        s = np.ascontiguousarray(seeds, np.uint32)
+        dist = np.empty(self.n, np.uint32) if copy_back else None
+        par = np.empty(self.n, np.uint32) if copy_back else None
+        st = N.PfStats()
+        N.check(self._lib.pf_multi_seed(self._h, N.ptr(s), len(s), C.byref(opt.to_c()), N.ptr(dist), N.ptr(par),
+                                        C.byref(st)))
+        return GpuRunResult(ms=st.ms, stats=GpuStats.from_c(st), dist=dist, parent=par)
+
+    def last_reach(self) -> tuple:
+        a, b = C.c_uint64(), C.c_uint64()
+        N.check(self._lib.pf_last_reach(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # ---- test hooks ----
+    def debug_light_end(self, delta: int) -> np.ndarray:
+        out = np.empty(self.n, np.uint32)
+        N.check(self._lib.pf_debug_light_end(self._h, delta, N.ptr(out)))
+        return out
+
+    def debug_heuristic(self, kind: Heuristic, target: int) -> np.ndarray:
+        out = np.empty(self.n, np.uint32)
+        N.check(self._lib.pf_debug_heuristic(self._h, int(kind), target, N.ptr(out)))
+        return out
+
+    def debug_sorted_rows(self):
+        col = np.empty(self.g.m, np.uint32)
+        w = np.empty(self.g.m, np.uint32)
+        N.check(self._lib.pf_debug_sorted_rows(self._h, N.ptr(col), N.ptr(w)))
+        return col, w
+
+
+def debug_exclusive_scan(a: np.ndarray):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty(max(a.shape[0], 1), np.uint32)
+    tot = C.c_uint32()
+    N.check(N.engine().pf_debug_exclusive_scan(N.ptr(a), N.ptr(out), a.shape[0], C.byref(tot)))
+    return out[: a.shape[0]], tot.value

@@ -1,0 +1,261 @@
+"""GPU parity: the HIP engine, through its C-ABI, against the oracle.
+
+Bar: bit-exact.  Every dist[v] equals Dijkstra's, every parent[v] is the
+smallest-index tight predecessor (the engine's packed (dist<<32|parent)
+minimum), every A* target distance and path equal the oracle's canonical A*.
+"""
+import numpy as np
+import pytest
+
+import pathfinding_gpu_amd as P
+from pathfinding_gpu_amd import AtomicMode as AM, GpuAlgo as GA, Mapping as MP, Drive as DR, Heuristic as HK
+
+pytestmark = pytest.mark.gpu
+
+INF = 0xFFFFFFFF
+
+
+def oracle_sssp(orc, g, s):
+    d, pf, pc, _, _ = orc.dijkstra(g.row_offsets, g.col_indices, g.weights, int(s))
+    return d, pc
+
+
+def assert_sssp_exact(orc, g, s, r, what=""):
+    d, pc = oracle_sssp(orc, g, s)
+    bad = np.flatnonzero(r.dist != d)
+    assert bad.size == 0, f"{what}: {bad.size} dist mismatches, first v={bad[0]} gpu={r.dist[bad[0]]} cpu={d[bad[0]]}"
+    badp = np.flatnonzero(r.parent != pc)
+    assert badp.size == 0, f"{what}: {badp.size} parent mismatches, first v={badp[0]} gpu={r.parent[badp[0]]} cpu={pc[badp[0]]}"
+
+
+@pytest.fixture(scope="module")
+def grid100():
+    return P.grid_road(100, 100, 42)
+
+
+@pytest.fixture(scope="module")
+def sgrid100(grid100):
+    s = P.GpuSolver(grid100, max_batch=8)
+    yield s
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def rmat12():
+    return P.rmat(12, 16, 11)
+
+
+@pytest.fixture(scope="module")
+def srmat12(rmat12):
+    s = P.GpuSolver(rmat12, max_batch=4)
+    yield s
+    s.close()
+
+
+# ---------------------------------------------------------------- config 1
+def test_config1_grid100_frontier_from_0(orc, grid100, sgrid100):
+    """100x100 grid road graph, seed 42, Frontier from vertex 0: dist + parent bit-exact."""
+    r = sgrid100.sssp(0, P.GpuOptions(algo=GA.Frontier))
+    assert_sssp_exact(orc, grid100, 0, r, "config1 frontier")
+    assert r.stats.edges_scanned >= grid100.m
+
+
+# ---------------------------------------------------------------- SSSP matrix
+ALGOS = [GA.NaiveRelax, GA.Frontier, GA.DeltaStep]
+MAPS = [MP.Thread, MP.Warp, MP.Hybrid]
+ATOMS = [AM.MinRBA, AM.CasRBA, AM.MinNoCheck]
+
+
+@pytest.mark.parametrize("algo", ALGOS, ids=lambda a: a.name)
+@pytest.mark.parametrize("mapping", MAPS, ids=lambda a: a.name)
+@pytest.mark.parametrize("atomic", ATOMS, ids=lambda a: a.name)
+def test_sssp_matrix_grid(orc, grid100, sgrid100, algo, mapping, atomic):
+    for s in (0, 4999, 9999):
+        r = sgrid100.sssp(s, P.GpuOptions(algo=algo, mapping=mapping, atomic=atomic))
+        assert_sssp_exact(orc, grid100, s, r, f"{algo.name}/{mapping.name}/{atomic.name} src {s}")
+
+
+@pytest.mark.parametrize("algo", ALGOS, ids=lambda a: a.name)
+@pytest.mark.parametrize("mapping", MAPS, ids=lambda a: a.name)
+def test_sssp_matrix_rmat(orc, rmat12, srmat12, algo, mapping):
+    src = rmat12.max_degree_vertex()
+    for s in (src, 1, 4000):
+        r = srmat12.sssp(s, P.GpuOptions(algo=algo, mapping=mapping))
+        assert_sssp_exact(orc, rmat12, s, r, f"{algo.name}/{mapping.name} src {s}")
+
+
+@pytest.mark.parametrize("drive", [DR.Host, DR.Graph, DR.Persistent], ids=lambda a: a.name)
+@pytest.mark.parametrize("sync_every", [1, 8])
+@pytest.mark.parametrize("algo", [GA.Frontier, GA.DeltaStep], ids=lambda a: a.name)
+def test_drives(orc, grid100, sgrid100, rmat12, srmat12, drive, sync_every, algo):
+    for g, sv, s in ((grid100, sgrid100, 123), (rmat12, srmat12, rmat12.max_degree_vertex())):
+        r = sv.sssp(s, P.GpuOptions(algo=algo, drive=drive, sync_every=sync_every))
+        assert_sssp_exact(orc, g, s, r, f"{drive.name}/S={sync_every}/{algo.name}")
+        if drive == DR.Persistent:
+            assert r.stats.launches == 1
+
+
+def test_prefix_scan_compaction(orc, grid100, sgrid100, rmat12, srmat12):
+    for g, sv, s in ((grid100, sgrid100, 77), (rmat12, srmat12, 3)):
+        for drive in (DR.Host, DR.Persistent):
+            r = sv.sssp(s, P.GpuOptions(algo=GA.Frontier, compaction=P.Compaction.PrefixScan, drive=drive))
+            assert_sssp_exact(orc, g, s, r, f"prefix-scan {drive.name}")
+
+
+@pytest.mark.parametrize("delta", [1, 37, 500, 5000, 1 << 30])
+def test_delta_values(orc, grid100, sgrid100, delta):
+    r = sgrid100.sssp(321, P.GpuOptions(delta=delta, drive=DR.Persistent))
+    assert_sssp_exact(orc, grid100, 321, r, f"delta {delta}")
+    assert r.stats.delta == delta
+
+
+@pytest.mark.parametrize("bs", [64, 128, 512, 1024])
+def test_block_sizes_and_cutoffs(orc, rmat12, srmat12, bs):
+    s = rmat12.max_degree_vertex()
+    for wc, bc in ((0, 0), (8, 64), (200, 4096)):
+        r = srmat12.sssp(s, P.GpuOptions(block_size=bs, wave_cutoff=wc, block_cutoff=bc))
+        assert_sssp_exact(orc, rmat12, s, r, f"bs {bs} cutoffs {wc}/{bc}")
+
+
+# ---------------------------------------------------------------- A*
+def oracle_astar(orc, g, s, t, kind):
+    return orc.astar(g.row_offsets, g.col_indices, g.weights, g.x, g.y, g.h_scale, int(kind), int(s), int(t), True)
+
+
+@pytest.mark.parametrize("kind", [HK.Scaled, HK.FloorEuclid, HK.None_], ids=lambda a: a.name)
+@pytest.mark.parametrize("algo", ALGOS, ids=lambda a: a.name)
+def test_astar_grid(orc, grid100, sgrid100, kind, algo):
+    rng = np.random.default_rng(int(kind) * 10 + int(algo))
+    for _ in range(4):
+        s, t = (int(v) for v in rng.integers(0, grid100.n, 2))
+        r = sgrid100.astar(s, t, P.GpuOptions(algo=algo, heuristic=kind))
+        d, path, _, _ = oracle_astar(orc, grid100, s, t, kind)
+        assert r.target_dist == d, (s, t)
+        assert np.array_equal(r.path, path), (s, t)
+
+
+def test_heuristic_values_exact(orc, grid100, sgrid100):
+    for t in (0, 5050, 9999):
+        for kind in (HK.Scaled, HK.FloorEuclid):
+            h = sgrid100.debug_heuristic(kind, t)
+            want = orc.heuristic_all(int(kind), grid100.x, grid100.y, grid100.h_scale, t)
+            assert np.array_equal(h, want), (kind.name, t)
+
+
+@pytest.mark.parametrize("drive", [DR.Host, DR.Persistent], ids=lambda a: a.name)
+def test_astar_batch(orc, grid100, sgrid100, drive):
+    rng = np.random.default_rng(5)
+    srcs = rng.integers(0, grid100.n, 8)
+    dsts = rng.integers(0, grid100.n, 8)
+    res = sgrid100.astar_batch(srcs, dsts, P.GpuOptions(heuristic=HK.FloorEuclid, drive=drive))
+    for i in range(8):
+        d, path, _, _ = oracle_astar(orc, grid100, srcs[i], dsts[i], HK.FloorEuclid)
+        assert res["dist"][i] == d
+        assert np.array_equal(res["paths"][i], path)
+
+
+# ---------------------------------------------------------------- batches
+@pytest.mark.parametrize("algo", [GA.Frontier, GA.DeltaStep], ids=lambda a: a.name)
+@pytest.mark.parametrize("drive", [DR.Host, DR.Graph, DR.Persistent], ids=lambda a: a.name)
+def test_multi_source(orc, grid100, sgrid100, algo, drive):
+    srcs = [0, 17, 4242, 9999, 5000, 123, 77, 8888]
+    r = sgrid100.multi_source(srcs, P.GpuOptions(algo=algo, drive=drive), parents=True)
+    for i, s in enumerate(srcs):
+        d, pc = oracle_sssp(orc, grid100, s)
+        assert np.array_equal(r.dist[i], d), f"slot {i}"
+        assert np.array_equal(r.parent[i], pc), f"slot {i}"
+
+
+def test_multi_source_prefix_scan(orc, rmat12, srmat12):
+    srcs = [rmat12.max_degree_vertex(), 5, 9, 4095]
+    r = srmat12.multi_source(srcs, P.GpuOptions(algo=GA.Frontier, compaction=P.Compaction.PrefixScan))
+    for i, s in enumerate(srcs):
+        assert np.array_equal(r.dist[i], oracle_sssp(orc, rmat12, s)[0])
+
+
+def test_multi_seed_union_of_grids(orc):
+    rng = np.random.default_rng(8)
+    gs = [P.grid_road(int(a), int(a), 1000 + i) for i, a in enumerate(rng.integers(8, 40, 12))]
+    u, voff = P.union(gs)
+    seeds = [int(voff[i]) + int(rng.integers(0, gs[i].n)) for i in range(len(gs))]
+    with P.GpuSolver(u) as sv:
+        for drive in (DR.Host, DR.Graph, DR.Persistent):
+            r = sv.multi_seed(seeds, P.GpuOptions(drive=drive))
+            for i, g in enumerate(gs):
+                sl = slice(int(voff[i]), int(voff[i + 1]))
+                d, pc = oracle_sssp(orc, g, seeds[i] - int(voff[i]))
+                assert np.array_equal(r.dist[sl], d), f"graph {i}"
+                par = r.parent[sl].astype(np.int64)
+                par[par != INF] -= int(voff[i])
+                assert np.array_equal(par, pc.astype(np.int64)), f"graph {i}"
+
+
+# ---------------------------------------------------------------- edge cases
+def test_edge_cases(orc):
+    # single vertex, no arcs
+    g = P.build_csr(1, [], [], [])
+    with P.GpuSolver(g) as sv:
+        r = sv.sssp(0, P.GpuOptions())
+        assert r.dist.tolist() == [0] and r.parent.tolist() == [INF]
+    # isolated source, unreachable part, zero weights (clamped to 1), self-loops, duplicates, near-INF weights
+    big = 0xFFFFFFF0
+    src = [1, 1, 2, 3, 3, 4, 4, 5, 6]
+    dst = [2, 2, 3, 3, 4, 5, 5, 6, 1]
+    w = [0, 5, 2, 9, big, 7, 3, 20, 1]
+    g = P.build_csr(8, src, dst, w)
+    with P.GpuSolver(g, max_batch=2) as sv:
+        for s in range(8):
+            for algo in ALGOS:
+                r = sv.sssp(s, P.GpuOptions(algo=algo))
+                assert_sssp_exact(orc, g, s, r, f"edge case src {s} {algo.name}")
+        with pytest.raises(RuntimeError):
+            sv.sssp(8, P.GpuOptions())
+        with pytest.raises(RuntimeError):
+            sv.multi_source([0, 1, 2], P.GpuOptions())   # k > max_batch
+
+
+def test_debug_hooks(orc, grid100, sgrid100, rmat12, srmat12):
+    for g, sv in ((grid100, sgrid100), (rmat12, srmat12)):
+        col, w = sv.debug_sorted_rows()
+        assert np.array_equal(col, g.col_indices) and np.array_equal(w, g.weights)
+        for delta in (1, 100, 1000):
+            assert np.array_equal(sv.debug_light_end(delta), orc.light_end(g.row_offsets, g.weights, delta))
+    from pathfinding_gpu_amd.solver import debug_exclusive_scan
+    a = np.random.default_rng(0).integers(0, 3, 100000).astype(np.uint32)
+    ex, tot = debug_exclusive_scan(a)
+    want, wt = orc.exclusive_scan(a)
+    assert tot == wt and np.array_equal(ex, want)
+
+
+def test_unsorted_rows_are_sorted_by_engine(orc):
+    rng = np.random.default_rng(3)
+    g = P.grid_road(20, 20, 9)
+    perm_g = P.CSRGraph(n=g.n, row_offsets=g.row_offsets.copy(), col_indices=g.col_indices.copy(),
+                        weights=g.weights.copy(), x=g.x, y=g.y, avg_w=g.avg_w, avg_deg=g.avg_deg, h_scale=g.h_scale)
+    for u in range(g.n):
+        b, e = g.row_offsets[u], g.row_offsets[u + 1]
+        p = rng.permutation(e - b) + b
+        perm_g.col_indices[b:e] = g.col_indices[p]
+        perm_g.weights[b:e] = g.weights[p]
+    import ctypes as C
+    from pathfinding_gpu_amd import _native as N
+    from pathfinding_gpu_amd.solver import _describe
+    d = _describe(perm_g)
+    d.flags = 1
+    h = C.c_void_p()
+    N.check(N.engine().pf_solver_create(C.byref(d), 1, C.byref(h)))
+    try:
+        col = np.empty(g.m, np.uint32)
+        w = np.empty(g.m, np.uint32)
+        N.check(N.engine().pf_debug_sorted_rows(h, N.ptr(col), N.ptr(w)))
+        assert np.array_equal(col, g.col_indices) and np.array_equal(w, g.weights)
+    finally:
+        N.engine().pf_solver_destroy(h)
+
+
+def test_reach_stats(orc, grid100, sgrid100):
+    sgrid100.sssp(0, P.GpuOptions(), copy_back=False)
+    reached, arcs = sgrid100.last_reach()
+    d, _ = oracle_sssp(orc, grid100, 0)
+    want_arcs, want_reached = orc.reachable_arcs(grid100.row_offsets, d)
+    assert (reached, arcs) == (want_reached, want_arcs)
