@@ -103,6 +103,7 @@ struct pf_solver {
     u32* d_slotDone = nullptr;
     u32* d_src = nullptr;
     Ctl* d_ctl = nullptr;
+    unsigned long long* d_bar = nullptr;  // persistent grid barrier: ticket and generation words
     unsigned long long* d_stats = nullptr;
     unsigned long long* d_reach = nullptr;
     u32* d_seeds = nullptr;
@@ -119,6 +120,7 @@ struct pf_solver {
     double uploadMs = 0.0;
     u64 bytes = 0;
     int cuCount = 1, maxThreadsPerCU = 2048;
+    double wallClockHz = 1e8;
     u32 lastK = 0;
     std::vector<GraphCacheEntry> graphs;
 
@@ -347,6 +349,8 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     run.tileItems = tileItems;
     run.numTiles = divUp(Kn, tileItems);
     run.total = (u32)Kn;
+    run.bar = s->d_bar;
+    run.spinTicks = (unsigned long long)(2.0 * s->wallClockHz);   // 2 s per barrier wait
 
     u32 startPhase = PH_LIGHT;
     u32 mode = 0;
@@ -380,7 +384,8 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         int nb = 0;
         PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, (int)bs, shm));
         const u32 maxCo = (u32)std::max(1, nb) * (u32)s->cuCount;
-        const u32 pg = std::max(1u, std::min(o.grid_blocks ? o.grid_blocks : maxCo, maxCo));
+        const u32 want = o.grid_blocks ? o.grid_blocks : 2u * (u32)s->cuCount;
+        const u32 pg = std::max(1u, std::min(want, maxCo));
         rc = launchStep(s, fn, pg, bs, shm, run, 1);
         if (rc) return rc;
         launches = 1;
@@ -427,7 +432,14 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     PF_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     PF_HIP(hipEventElapsedTime(&loopMs, s->evLoop, s->ev1));
     const Ctl& hc = *s->h_ctl;
-    if (hc.error & E_BARRIER) return fail(PF_ERR_HIP, "grid barrier wait exceeded its spin limit");
+    if (hc.error & E_BARRIER) {
+        unsigned long long b[BAR_GEN + 1] = {};
+        (void)hipMemcpy(b, s->d_bar, sizeof(b), hipMemcpyDeviceToHost);
+        char buf[200];
+        std::snprintf(buf, sizeof(buf), "grid barrier wait exceeded its spin limit (tickets %llu, released %llu, phase %u, steps %u)",
+                      b[0], b[BAR_GEN], hc.phase, hc.steps);
+        return fail(PF_ERR_HIP, buf);
+    }
     if (hc.error & E_NOT_CONVERGED || hc.phase != PH_DONE) return fail(PF_ERR_NOT_CONVERGED, "relaxation did not converge");
     s->lastK = spec.k;
     if (st) {
@@ -625,6 +637,11 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
     if (hipGetDeviceProperties(&p, dev) != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "device properties"));
     s->cuCount = p.multiProcessorCount;
     s->maxThreadsPerCU = p.maxThreadsPerMultiProcessor;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            s->wallClockHz = khz * 1e3;
+    }
     if (hipStreamCreate(&s->s) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
         hipEventCreate(&s->ev1) != hipSuccess || hipEventCreate(&s->evLoop) != hipSuccess)
         return cleanupFail(fail(PF_ERR_HIP, "stream/event create"));
@@ -707,10 +724,12 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
         (rc = s->dalloc(&s->d_rMask, maskWords(Kn))) || (rc = s->dalloc(&s->d_bound, K)) ||
         (rc = s->dalloc(&s->d_newBound, K)) || (rc = s->dalloc(&s->d_minKey, K)) ||
         (rc = s->dalloc(&s->d_target, K)) || (rc = s->dalloc(&s->d_slotDone, K)) ||
-        (rc = s->dalloc(&s->d_src, K)) || (rc = s->dalloc(&s->d_ctl, 1)) || (rc = s->dalloc(&s->d_stats, 5)) ||
+        (rc = s->dalloc(&s->d_src, K)) || (rc = s->dalloc(&s->d_stats, 5)) ||
         (rc = s->dalloc(&s->d_reach, 2)) || (rc = s->dalloc(&s->d_lens, K)) || (rc = s->dalloc(&s->d_tdist, K)))
         return cleanupFail(rc);
-    if (hipMemset(s->d_ctl, 0, sizeof(Ctl)) != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "memset"));
+    if ((rc = s->dalloc(&s->d_ctl, 1)) || (rc = s->dalloc(&s->d_bar, 256))) return cleanupFail(rc);
+    if (hipMemset(s->d_ctl, 0, sizeof(Ctl)) != hipSuccess || hipMemset(s->d_bar, 0, 256 * 8) != hipSuccess)
+        return cleanupFail(fail(PF_ERR_HIP, "memset"));
     if (hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)
         return cleanupFail(fail(PF_ERR_NO_MEMORY, "pinned ctl"));
     if (hipDeviceSynchronize() != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "sync"));
@@ -728,7 +747,7 @@ void pf_solver_destroy(pf_solver* s) {
     void* ptrs[] = {s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
-                    s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
+                    s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
                     s->d_lens, s->d_tdist};
     for (void* p : ptrs)
         if (p) hipFree(p);

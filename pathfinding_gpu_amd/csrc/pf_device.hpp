@@ -102,7 +102,11 @@ struct Run {
     u32 waveCutoff, blockCutoff;
     u32 numTiles, tileItems;     // prefix-scan tiling
     u32 total;                   // K*n
+    unsigned long long* bar;     // persistent grid barrier: [0] ticket, [BAR_GEN] released barriers
+    unsigned long long spinTicks;  // wall-clock bound of one barrier wait
 };
+
+constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
 
 struct LocalStats {
     u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
@@ -122,6 +126,12 @@ __device__ __forceinline__ u32 ldAgent(const u32* p) {
 __device__ __forceinline__ void stAgent(u32* p, u32 v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned long long ldAgent64(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stAgent64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ u32 laneId() { return __lane_id(); }
 
@@ -137,7 +147,7 @@ __device__ __forceinline__ u32 slotOf(const Run& r, u32 gi) {
 // Wave-aggregated increment: one atomicAdd per group of active lanes
 // (replaces warpAggregatedInc, src/gpu_kernels.cuh:135-141, for 64 lanes).
 __device__ __forceinline__ u32 waveAggInc(u32* ctr) {
-    const u64 active = __builtin_amdgcn_read_exec();
+    const u64 active = __builtin_amdgcn_ballot_w64(true);   // lanes taking part (convergent builtin)
     const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(active >> 32), __builtin_amdgcn_mbcnt_lo((u32)active, 0u));
     u32 base = 0;
     if (rank == 0) base = atomicAdd(ctr, (u32)__popcll(active));
@@ -725,12 +735,11 @@ __device__ __forceinline__ void flushStats(unsigned long long* stats, const Loca
     }
 }
 
-constexpr u32 SPIN_LIMIT = 1u << 25;   // bounded barrier wait (about a few seconds)
-
 template <bool ASTAR, int AM, int MAP>
 __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
     extern __shared__ GatherSlot gather[];
-    __shared__ u32 s_phase, s_in, s_fcur, s_cur, s_size, s_last, s_gen, s_exit;
+    __shared__ u32 s_phase, s_in, s_fcur, s_cur, s_size, s_last, s_exit;
+    __shared__ unsigned long long s_bar;
     __shared__ u32 s_owner;
     __shared__ Item s_item;
     __shared__ u32 waveTotals[17];
@@ -794,34 +803,43 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             default: break;
         }
         // ---- arrive; the last workgroup runs the controller ----
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // every wave's stores, before the ticket
+        // Each wave's stores complete at the workgroup barrier; thread 0 then publishes them
+        // device-wide (agent release) and takes a ticket.  In persistent mode the ticket is a
+        // monotonic 64-bit counter: barrier number and "last" follow from the ticket alone, and
+        // nothing resets it while other workgroups may still be adding to it.
         __syncthreads();
         if (threadIdx.x == 0) {
-            s_gen = ldAgent(&c->gen);
-            const u32 t = atomicAdd(&c->arrive, 1u);
-            s_last = (t == gridDim.x - 1) ? 1u : 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (persistent) {
+                const unsigned long long t = atomicAdd(&r.bar[0], 1ull);
+                s_bar = t / gridDim.x;
+                s_last = (t % gridDim.x == gridDim.x - 1) ? 1u : 0u;
+            } else {
+                const u32 t = atomicAdd(&c->arrive, 1u);
+                s_last = (t == gridDim.x - 1) ? 1u : 0u;
+            }
+            if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (s_last) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             controller(r, waveTotals);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
             if (threadIdx.x == 0) {
-                stAgent(&c->arrive, 0u);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                stAgent(&c->gen, s_gen + 1u);
+                if (persistent) stAgent64(&r.bar[BAR_GEN], s_bar + 1ull);
+                else stAgent(&c->arrive, 0u);   // next launch starts after this kernel ends
             }
         } else if (persistent) {
             if (threadIdx.x == 0) {
-                u32 spins = 0;
-                while (ldAgent(&c->gen) == s_gen) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > SPIN_LIMIT) { atomicOr(&c->error, (u32)E_BARRIER); break; }
+                // relaxed agent-scope polling, bounded by wall-clock time; one acquire after the wait
+                const unsigned long long t0 = wall_clock64();
+                while (ldAgent64(&r.bar[BAR_GEN]) <= s_bar) {
+                    __builtin_amdgcn_s_sleep(8);
+                    if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); break; }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         if (!persistent) break;
         if (threadIdx.x == 0) s_exit = ldAgent(&c->pause) | (ldAgent(&c->error) & E_BARRIER);
@@ -885,6 +903,8 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         for (int j = 0; j < C_COUNT; ++j) c->cnt[j] = 0;
         c->cnt[C_NEAR0] = mode == 0 ? nseeds : 0u;
         c->arrive = 0; c->gen = 0;
+        r.bar[0] = 0ull;
+        r.bar[BAR_GEN] = 0ull;
         c->steps = 0; c->iters = 0; c->buckets = 0; c->error = 0;
         c->scanTotal = 0; c->epochEnd = 0; c->pause = 0; c->maxSteps = maxSteps;
         r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
