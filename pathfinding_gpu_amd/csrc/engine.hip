@@ -362,7 +362,9 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
 
     const u32 shm = (bs / 64) * 64 * sizeof(GatherSlot);
     const StepFn fn = stepTable[spec.astar ? 1 : 0][o.atomic][o.mapping];
-    u32 grid = o.grid_blocks ? o.grid_blocks : defaultGrid(s, bs, Kn);
+    // steps launched from the host: one wave of workgroups per CU at most (the ticket of every
+    // workgroup is on the step's critical path)
+    u32 grid = o.grid_blocks ? o.grid_blocks : std::min(defaultGrid(s, bs, Kn), (u32)s->cuCount);
     u32 launches = 0, syncs = 0;
 
     PF_HIP(hipEventRecord(s->ev0, s->s));   // ===== timed region (src/gpu_sssp.cu:478-502) =====
@@ -410,7 +412,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
             } else {
                 for (u32 i = 0; i < S; ++i) {
                     const u32 g = (S == 1 && !o.grid_blocks)
-                                      ? std::max(1u, std::min(defaultGrid(s, bs, Kn), inputSize(hc, s->n, run.numTiles, bs)))
+                                      ? std::max(1u, std::min(grid, inputSize(hc, s->n, run.numTiles, bs)))
                                       : grid;
                     rc = launchStep(s, fn, g, bs, shm, run, 0);
                     if (rc) return rc;

@@ -56,14 +56,15 @@ enum CtlError : u32 { E_NONE = 0, E_NOT_CONVERGED = 1, E_BARRIER = 2 };
 
 // Control block (HBM).  Fields are read by every workgroup at the start of a
 // step and written only by the controller (the step's last workgroup).
-struct Ctl {
+struct alignas(16) Ctl {
     u32 phase, in, fcur, cur;   // phase, near input buffer, current far buffer, naive flag buffer
     u32 cnt[C_COUNT];           // queue sizes, naive "changed"
     u32 arrive, gen;            // step ticket / barrier generation
     u32 steps, iters, buckets, error;
     u32 scanTotal, epochEnd, pause, maxSteps;
-    u32 pad[8];
+    u32 pad[10];
 };
+static_assert(sizeof(Ctl) == 128, "control block is eight 16-byte words");
 
 struct DevGraph {
     const u32* row;       // row_offsets [n+1]
@@ -584,87 +585,88 @@ __device__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
 // ---------------------------------------------------------------------------
 // controller: the host loop decisions of src/gpu_sssp.cu:257-456, on device
 // ---------------------------------------------------------------------------
-struct CtlView {
-    u32 phase, in, fcur, cur;
-};
-
+// The controller runs in one workgroup after every workgroup of the step has
+// arrived.  It reads the control block with a few wide loads into LDS, decides
+// the next phase there, and writes the block back with a few wide stores, so its
+// critical path is a handful of memory round trips.
 __device__ void controller(const Run& r, u32* waveTotals) {
     Ctl* c = r.ctl;
-    __shared__ u32 s_next, s_any, s_resetMin, s_setBounds, s_scan;
+    __shared__ __align__(16) Ctl h;           // working copy
+    __shared__ u32 s_next, s_resetMin, s_setBounds, s_scan, s_part;
+    if (threadIdx.x < sizeof(Ctl) / 16) reinterpret_cast<uint4*>(&h)[threadIdx.x] = reinterpret_cast<const uint4*>(c)[threadIdx.x];
+    __syncthreads();
     if (threadIdx.x == 0) {
-        u32 phase = c->phase;
-        u32 in = c->in, fcur = c->fcur;
+        const u32 phase = h.phase;
+        u32 in = h.in, fcur = h.fcur;
         u32 next = PH_DONE;
         u32 resetMin = 0, setBounds = 0, scan = 0;
         auto bucketDone = [&]() {
-            c->buckets++;
-            if (c->cnt[C_FAR0 + fcur] == 0) next = PH_DONE;
+            h.buckets++;
+            if (h.cnt[C_FAR0 + fcur] == 0) next = PH_DONE;
             else { next = PH_FARMIN; resetMin = 1; }
         };
         switch (phase) {
-            case PH_LIGHT: {
-                if (c->cnt[C_NEAR0 + in]) c->iters++;
+            case PH_LIGHT:
+                if (h.cnt[C_NEAR0 + in]) h.iters++;
+                h.cnt[C_NEAR0 + in] = 0;        // consumed input: next step's output buffer
                 in ^= 1u;
-                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else if (c->cnt[C_R]) { next = PH_HEAVY; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else { c->cnt[C_NEAR0 + (in ^ 1u)] = 0; bucketDone(); }
-                break;
-            }
-            case PH_HEAVY: {
-                c->iters++;
-                c->cnt[C_R] = 0;
-                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                if (h.cnt[C_NEAR0 + in]) next = PH_LIGHT;
+                else if (h.cnt[C_R]) next = PH_HEAVY;
                 else bucketDone();
                 break;
-            }
-            case PH_FARMIN: next = PH_PART; setBounds = 1; break;   // decided below over all slots
-            case PH_PART: {
-                c->cnt[C_FAR0 + fcur] = 0;
+            case PH_HEAVY:
+                h.iters++;
+                h.cnt[C_R] = 0;
+                if (h.cnt[C_NEAR0 + in]) { next = PH_LIGHT; h.cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else bucketDone();
+                break;
+            case PH_FARMIN: next = PH_PART; setBounds = 1; break;      // decided over all slots below
+            case PH_PART:
+                h.cnt[C_FAR0 + fcur] = 0;
                 fcur ^= 1u;
-                if (c->cnt[C_NEAR0 + in]) { next = PH_LIGHT; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else if (c->cnt[C_FAR0 + fcur]) { next = PH_FARMIN; resetMin = 1; }
+                if (h.cnt[C_NEAR0 + in]) { next = PH_LIGHT; h.cnt[C_NEAR0 + (in ^ 1u)] = 0; }
+                else if (h.cnt[C_FAR0 + fcur]) { next = PH_FARMIN; resetMin = 1; }
                 else next = PH_DONE;
                 break;
-            }
-            case PH_FRONTIER: {
-                if (c->cnt[C_NEAR0 + in]) c->iters++;
+            case PH_FRONTIER:
+                if (h.cnt[C_NEAR0 + in]) h.iters++;
+                h.cnt[C_NEAR0 + in] = 0;
                 in ^= 1u;
-                if (c->cnt[C_NEAR0 + in]) { next = PH_FRONTIER; c->cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else next = PH_DONE;
+                next = h.cnt[C_NEAR0 + in] ? PH_FRONTIER : PH_DONE;
                 break;
-            }
-            case PH_NAIVE: {
-                c->iters++;
-                c->cur ^= 1u;
-                if (c->cnt[C_CHANGED]) { next = PH_NAIVE; c->cnt[C_CHANGED] = 0; }
-                else next = PH_DONE;
+            case PH_NAIVE:
+                h.iters++;
+                h.cur ^= 1u;
+                if (h.cnt[C_CHANGED]) { next = PH_NAIVE; h.cnt[C_CHANGED] = 0; }
                 break;
-            }
-            case PH_SCAN_RELAX: {
-                if (c->cnt[C_NEAR0 + in]) c->iters++;
+            case PH_SCAN_RELAX:
+                if (h.cnt[C_NEAR0 + in]) h.iters++;
                 next = PH_SCAN_TILES;
                 break;
-            }
             case PH_SCAN_TILES: next = PH_SCAN_SCATTER; scan = 1; break;
-            case PH_SCAN_SCATTER: {
-                in ^= 1u;
-                c->cnt[C_NEAR0 + in] = c->scanTotal;
-                c->cnt[C_NEAR0 + (in ^ 1u)] = 0;
-                next = c->scanTotal ? PH_SCAN_RELAX : PH_DONE;
-                break;
-            }
+            case PH_SCAN_SCATTER: break;        // total known after the tile scan below
             default: next = PH_DONE;
         }
-        c->in = in;
-        c->fcur = fcur;
+        h.in = in;
+        h.fcur = fcur;
         s_next = next;
         s_resetMin = resetMin;
         s_setBounds = setBounds;
         s_scan = scan;
+        s_part = phase == PH_PART;
+        if (phase == PH_SCAN_SCATTER) {
+            h.in ^= 1u;
+            h.cnt[C_NEAR0 + h.in] = h.scanTotal;
+            h.cnt[C_NEAR0 + (h.in ^ 1u)] = 0;
+            s_next = h.scanTotal ? PH_SCAN_RELAX : PH_DONE;
+        }
     }
     __syncthreads();
     if (s_resetMin) {
         for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.minKey[k] = NOKEY;
+    }
+    if (s_part) {   // the partition has used old and new bounds; the new ones take over
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.bound[k] = r.newBound[k];
     }
     if (s_setBounds) {
         bool any = false;
@@ -680,39 +682,42 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         }
         const int anyAll = __syncthreads_or(any);
         if (threadIdx.x == 0) {
-            if (anyAll) c->cnt[C_FAR0 + (c->fcur ^ 1u)] = 0;
+            if (anyAll) h.cnt[C_FAR0 + (h.fcur ^ 1u)] = 0;
             else s_next = PH_DONE;
         }
     }
-    if (s_scan) {   // exclusive scan of tile sums in place
+    if (s_scan) {   // exclusive scan of the tile sums, in place
         const u32 nt = r.numTiles;
         const u32 per = (nt + blockDim.x - 1) / blockDim.x;
         const u32 b = threadIdx.x * per;
-        u32 s = 0;
+        u32 sum = 0;
         for (u32 j = 0; j < per; ++j)
-            if (b + j < nt) s += r.tileSums[b + j];
+            if (b + j < nt) sum += r.tileSums[b + j];
         u32 ex;
-        const u32 tot = blockExclusiveScan(s, waveTotals, ex);
+        const u32 tot = blockExclusiveScan(sum, waveTotals, ex);
         for (u32 j = 0; j < per; ++j)
             if (b + j < nt) { const u32 v = r.tileSums[b + j]; r.tileSums[b + j] = ex; ex += v; }
-        if (threadIdx.x == 0) c->scanTotal = tot;
+        if (threadIdx.x == 0) h.scanTotal = tot;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        // bounds move forward only once the partition has used old and new
-        if (c->phase == PH_PART) { /* handled below */ }
         u32 next = s_next;
-        c->steps++;
-        if (c->steps >= c->maxSteps && next != PH_DONE) { c->error = E_NOT_CONVERGED; next = PH_DONE; }
-        if (c->epochEnd && c->steps >= c->epochEnd && next != PH_DONE) c->pause = 1u;
-        s_next = next;
+        h.steps++;
+        if (h.steps >= h.maxSteps && next != PH_DONE) { atomicOr(&c->error, (u32)E_NOT_CONVERGED); next = PH_DONE; }
+        if (h.epochEnd && h.steps >= h.epochEnd && next != PH_DONE) h.pause = 1u;
+        h.phase = next;
+        // write back: header and counters (words 0-11), statistics, scan total, pause
+        uint4* dst = reinterpret_cast<uint4*>(c);
+        const uint4* src = reinterpret_cast<const uint4*>(&h);
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
+        c->steps = h.steps;
+        c->iters = h.iters;
+        c->buckets = h.buckets;
+        c->scanTotal = h.scanTotal;
+        c->pause = h.pause;
     }
-    __syncthreads();
-    if (c->phase == PH_PART) {
-        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.bound[k] = r.newBound[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) c->phase = s_next;
 }
 
 // ---------------------------------------------------------------------------
