@@ -160,6 +160,27 @@ int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h); /*
 int pf_debug_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total);
 int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w); /* the uploaded CSR, m each */
 
+/* ---- learned Delta: a small MLP with HIP inference / training kernels --------
+ * No reference counterpart (the reference has only the Davidson default,
+ * src/gpu_sssp.cu:202-206).  Network in -> H -> H -> out (ReLU, FP32); widths
+ * <= 64.  Parameters are one flat array: W1[H][in] b1[H] W2[H][H] b2[H]
+ * W3[out][H] b3[out].  Loss: masked mean squared error. */
+typedef struct pf_mlp pf_mlp;
+const char* pf_mlp_last_error(void);
+int pf_mlp_create(uint32_t n_in, uint32_t n_hidden, uint32_t n_out, uint64_t seed, pf_mlp** out);
+void pf_mlp_destroy(pf_mlp* m);
+uint32_t pf_mlp_num_params(const pf_mlp* m);
+int pf_mlp_get_params(pf_mlp* m, float* params);
+int pf_mlp_set_params(pf_mlp* m, const float* params);
+/* y[n][out] = f(x[n][in]) on the GPU */
+int pf_mlp_forward(pf_mlp* m, const float* x, uint32_t n, float* y);
+/* loss and its gradient w.r.t. the parameters (GPU forward + backward) */
+int pf_mlp_loss_grad(pf_mlp* m, const float* x, const float* t, const float* mask, uint32_t n, float* loss,
+                     float* grad);
+/* `epochs` full-batch Adam steps on the GPU; *final_loss after the last step */
+int pf_mlp_train(pf_mlp* m, const float* x, const float* t, const float* mask, uint32_t n, uint32_t epochs,
+                 float lr, float* final_loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,17 @@ ENGINE_SYMBOLS = {
     "pf_debug_heuristic": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, u32p]),
     "pf_debug_exclusive_scan": (C.c_int, [u32p, u32p, C.c_uint32, u32p]),
     "pf_debug_sorted_rows": (C.c_int, [C.c_void_p, u32p, u32p]),
+    "pf_mlp_last_error": (C.c_char_p, []),
+    "pf_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pf_mlp_destroy": (None, [C.c_void_p]),
+    "pf_mlp_num_params": (C.c_uint32, [C.c_void_p]),
+    "pf_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pf_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pf_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float)]),
+    "pf_mlp_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "pf_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                               C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
 }
 
 GRAPH_SYMBOLS = {

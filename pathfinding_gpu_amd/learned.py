@@ -1,0 +1,245 @@
+"""Learned Delta: a small MLP predicts the Delta-stepping bucket width and the mapping regime of a graph.
+
+No reference counterpart: the reference always uses the Davidson default
+Delta = 32 * avgW / avgDeg (src/gpu_sssp.cu:202-206) and a fixed mapping.  Here a
+network trained on timings of this engine's own Delta sweep (delta_sweep.py,
+the counterpart of the reference's runDeltaStudy, src/main.cpp:560-608) scores
+candidate Deltas (multiples of the Davidson default) and the three mappings from
+graph statistics and the batch size.  Inference and training run on the GPU
+through the hand-written kernels of csrc/mlp.hip.
+
+Delta only reschedules work: every Delta gives bit-identical distances and
+parents.  The Davidson default stays the fallback and the judge: `Tuner.choose`
+times the predicted configuration against the default once per graph and keeps
+the default when the prediction is slower.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .graph import CSRGraph
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+MODEL_PATH = os.path.join(DATA, "delta_mlp.npz")
+
+#: candidate Deltas as multiples of the Davidson default
+DELTA_MULTS = np.array([2.0 ** e for e in range(-5, 6)], np.float64)
+#: mapping regimes scored by the network (values of pf_mapping)
+MAPPINGS = (0, 1, 2)
+N_OUT = len(DELTA_MULTS) + len(MAPPINGS)
+FEATURES = ("log2_n", "log2_avg_deg", "log2_max_deg", "log2_avg_w", "cv_w", "cv_deg", "log2_w_range",
+            "has_coords", "log2_batch", "frac_deg_le4")
+
+
+def davidson_delta(avg_w: float, avg_deg: float) -> int:
+    """autoDelta (src/gpu_sssp.cu:202-206)."""
+    return max(1, int(math.floor(32.0 * avg_w / max(1.0, avg_deg) + 0.5)))
+
+
+def graph_features(g: CSRGraph, batch: int = 1) -> np.ndarray:
+    deg = np.diff(g.row_offsets).astype(np.float64)
+    w = g.weights.astype(np.float64)
+    mw = w.mean() if w.size else 1.0
+    md = deg.mean() if deg.size else 1.0
+    return np.array([
+        math.log2(max(g.n, 1)),
+        math.log2(max(g.avg_deg, 1e-3)),
+        math.log2(g.max_deg + 1.0),
+        math.log2(max(g.avg_w, 1e-3)),
+        float(w.std() / mw) if w.size else 0.0,
+        float(deg.std() / md) if md > 0 else 0.0,
+        math.log2(max(g.max_w, 1) / max(g.min_w, 1)),
+        1.0 if g.has_coords() else 0.0,
+        math.log2(max(batch, 1)),
+        float((deg <= 4).mean()) if deg.size else 1.0,
+    ], np.float64)
+
+
+class Mlp:
+    """Handle on a GPU-resident network (csrc/mlp.hip)."""
+
+    def __init__(self, n_in: int, n_hidden: int, n_out: int, seed: int = 1):
+        self.dims = (n_in, n_hidden, n_out)
+        self._lib = N.engine()
+        h = C.c_void_p()
+        rc = self._lib.pf_mlp_create(n_in, n_hidden, n_out, seed, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pf_mlp_create: {self._lib.pf_mlp_last_error().decode()}")
+        self._h = h
+        self.P = self._lib.pf_mlp_num_params(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pf_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"mlp: {self._lib.pf_mlp_last_error().decode()}")
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.P, np.float32)
+        self._chk(self._lib.pf_mlp_get_params(self._h, N.ptr(p, C.c_float)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.P
+        self._chk(self._lib.pf_mlp_set_params(self._h, N.ptr(p, C.c_float)))
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], self.dims[2]), np.float32)
+        self._chk(self._lib.pf_mlp_forward(self._h, N.ptr(X, C.c_float), X.shape[0], N.ptr(Y, C.c_float)))
+        return Y
+
+    def loss_grad(self, X, T, M) -> Tuple[float, np.ndarray]:
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        g = np.empty(self.P, np.float32)
+        l = C.c_float()
+        self._chk(self._lib.pf_mlp_loss_grad(self._h, N.ptr(X, C.c_float), N.ptr(T, C.c_float), N.ptr(M, C.c_float),
+                                             X.shape[0], C.byref(l), N.ptr(g, C.c_float)))
+        return l.value, g
+
+    def train(self, X, T, M, epochs: int, lr: float) -> float:
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        l = C.c_float()
+        self._chk(self._lib.pf_mlp_train(self._h, N.ptr(X, C.c_float), N.ptr(T, C.c_float), N.ptr(M, C.c_float),
+                                         X.shape[0], epochs, lr, C.byref(l)))
+        return l.value
+
+
+@dataclass
+class DeltaModel:
+    params: np.ndarray
+    mu: np.ndarray
+    sd: np.ndarray
+    hidden: int = 32
+
+    @staticmethod
+    def load(path: str = MODEL_PATH) -> "DeltaModel":
+        z = np.load(path, allow_pickle=False)
+        return DeltaModel(z["params"], z["mu"], z["sd"], int(z["hidden"]))
+
+    def save(self, path: str = MODEL_PATH, **meta) -> None:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, params=self.params, mu=self.mu, sd=self.sd, hidden=np.int64(self.hidden),
+                 mults=DELTA_MULTS, **{k: np.asarray(v) for k, v in meta.items()})
+
+    def predict(self, feats: np.ndarray) -> np.ndarray:
+        """Scores (predicted log2 time relative to the best) for every output; GPU forward."""
+        net = Mlp(len(FEATURES), self.hidden, N_OUT)
+        try:
+            net.set_params(self.params)
+            return net.forward(((np.atleast_2d(feats) - self.mu) / self.sd).astype(np.float32))
+        finally:
+            net.close()
+
+    def choose(self, g: CSRGraph, batch: int = 1) -> Dict[str, int]:
+        s = self.predict(graph_features(g, batch))[0]
+        auto = davidson_delta(g.avg_w, g.avg_deg)
+        i = int(np.argmin(s[: len(DELTA_MULTS)]))
+        j = int(np.argmin(s[len(DELTA_MULTS):]))
+        return {"delta": max(1, int(round(auto * DELTA_MULTS[i]))), "mapping": MAPPINGS[j], "mult_index": i,
+                "auto_delta": auto}
+
+
+def train_model(X: np.ndarray, Y: np.ndarray, M: np.ndarray, hidden: int = 32, epochs: int = 3000,
+                lr: float = 3e-3, seed: int = 1) -> Tuple[DeltaModel, float]:
+    """Full-batch Adam on the GPU; features standardised with the training statistics."""
+    mu = X.mean(0)
+    sd = X.std(0)
+    sd[sd < 1e-6] = 1.0
+    net = Mlp(X.shape[1], hidden, Y.shape[1], seed)
+    try:
+        loss = net.train((X - mu) / sd, Y, M, epochs, lr)
+        return DeltaModel(net.params(), mu, sd, hidden), loss
+    finally:
+        net.close()
+
+
+class Tuner:
+    """Picks Delta and mapping for a graph: MLP prediction, judged once against the Davidson default."""
+
+    def __init__(self, model: Optional[DeltaModel] = None):
+        if model is None and os.path.exists(MODEL_PATH):
+            model = DeltaModel.load()
+        self.model = model
+        self.cache: Dict[tuple, dict] = {}
+
+    def choose(self, solver, g: CSRGraph, batch: int, time_fn) -> dict:
+        """time_fn(delta, mapping) -> milliseconds of one representative run on `solver`."""
+        key = (g.name, g.n, g.m, batch)
+        if key in self.cache:
+            return self.cache[key]
+        auto = davidson_delta(g.avg_w, g.avg_deg)
+        res = {"delta": auto, "mapping": 2, "source": "davidson", "auto_delta": auto}
+        if self.model is not None:
+            pick = self.model.choose(g, batch)
+            t_def = time_fn(auto, 2)
+            t_new = time_fn(pick["delta"], pick["mapping"])
+            res.update({"default_ms": t_def, "predicted_ms": t_new, "predicted_delta": pick["delta"],
+                        "predicted_mapping": pick["mapping"]})
+            if t_new < t_def:
+                res.update({"delta": pick["delta"], "mapping": pick["mapping"], "source": "mlp"})
+            else:
+                res["source"] = "davidson (prediction dropped: slower)"
+        self.cache[key] = res
+        return res
+
+
+def evaluate(model: DeltaModel, X: np.ndarray, Y: np.ndarray) -> dict:
+    """Regret (log2 of chosen time over best time) of the network's Delta and of the Davidson default."""
+    S = model.predict(X)
+    nd = len(DELTA_MULTS)
+    pick = np.argmin(S[:, :nd], axis=1)
+    dav = int(np.flatnonzero(DELTA_MULTS == 1.0)[0])
+    r_mlp = Y[np.arange(len(Y)), pick]
+    r_dav = Y[:, dav]
+    return {"mean_regret_mlp": float(r_mlp.mean()), "mean_regret_davidson": float(r_dav.mean()),
+            "speedup_geomean": float(2.0 ** (r_dav - np.minimum(r_mlp, r_dav)).mean()),
+            "picks": pick.tolist()}
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train the learned-Delta network on a delta_sweep dataset (GPU)")
+    ap.add_argument("dataset")
+    ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--hidden", type=int, default=32)
+    a = ap.parse_args(argv)
+    z = np.load(a.dataset, allow_pickle=False)
+    X, Y, M = z["X"], z["Y"], z["M"]
+    names = [str(s) for s in z["names"]]
+    # leave-one-graph-out check of generalisation (graph name without the batch suffix)
+    groups = sorted({n.split("|")[0] for n in names})
+    held = []
+    for gname in groups:
+        tr = np.array([n.split("|")[0] != gname for n in names])
+        m, _ = train_model(X[tr], Y[tr], M[tr], a.hidden, a.epochs // 4, a.lr)
+        ev = evaluate(m, X[~tr], Y[~tr])
+        held.append((ev["mean_regret_mlp"], ev["mean_regret_davidson"]))
+    model, loss = train_model(X, Y, M, a.hidden, a.epochs, a.lr)
+    ev = evaluate(model, X, Y)
+    h = np.array(held)
+    model.save(a.out, train_loss=loss, n_samples=len(X), holdout_regret_mlp=h[:, 0].mean(),
+               holdout_regret_davidson=h[:, 1].mean())
+    print(f"[train] samples={len(X)} loss={loss:.4f} train regret mlp={ev['mean_regret_mlp']:.3f} "
+          f"davidson={ev['mean_regret_davidson']:.3f} | leave-one-graph-out regret mlp={h[:, 0].mean():.3f} "
+          f"davidson={h[:, 1].mean():.3f} -> {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
