@@ -1,20 +1,26 @@
 """Benchmark of the SSSP / A* engine on one MI355X.
 
-Workload (the headline configuration): 1000x1000 grid road graph, seed 7
-(1,000,000 vertices, 3,996,000 arcs); Delta-stepping SSSP from vertex 0, plus
-a batch of 256 point-to-point A* queries with uniform random endpoints
-(h = floor(Euclidean distance)).
+Workload (default, --workload grid1000): the 1000x1000 grid road graph, seed 7
+(1,000,000 vertices, 3,996,000 arcs, integer jittered coordinates,
+w = ceil(len * U[1, 1.5))), Delta-stepping from vertex 0, plus 256 point-to-point
+A* queries with uniform random endpoints (h = floor(Euclidean distance)).
+--workload rmat20 runs the scale-20 R-MAT (edge factor 16, seed 11, source = the
+highest-degree vertex) the same way, without A*.
 
-One step = one SSSP from vertex 0 on the tuned, device-resident engine
-(persistent kernel, tuned Delta).  `value` is its GTEPS: reachable arcs
-(the reference's TEPS numerator, src/main.cpp:209-213) / wall time of the step,
-with the graph already resident in HBM and no copy back.  Reported beside it:
-the plain path (Davidson default Delta, host-driven loop, sync every 8
-launches, as the reference), the 256-query A* batch (queries/s and per-query
-latency), the roofline of the step kernel, and the reference's cpuDijkstra
-timed on this host.
+One step = one batched run of K = 64 single-source shortest-path queries
+(vertex 0 and 63 random sources) in one persistent launch, with the Delta and
+mapping the learned network picks for this graph and batch size (judged against
+the Davidson default).  `value` = GTEPS of that step: reachable arcs summed over
+the K queries (the reference's TEPS numerator, src/main.cpp:209-213) / wall time,
+graph resident in HBM, no copy back.  Reported beside it:
+  * plain: one query from vertex 0 with the Davidson default Delta (persistent
+    kernel), and the reference-style host-driven loop (sync every 8 launches);
+  * learned single-query: one query from vertex 0 with the learned Delta;
+  * A*: the 256-query batch (queries/s, latency) and plain one-at-a-time A*;
+  * roofline of the step kernel; the reference's cpuDijkstra (1 core) and
+    cpuMultiSource (all cores) on this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--workload grid1000|rmat20]
 """
 import argparse
 import json
@@ -34,15 +40,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def timed(fn, reps):
+    ts, r = [], None
+    for _ in range(reps):
+        a = time.perf_counter()
+        r = fn()
+        ts.append((time.perf_counter() - a) * 1e3)
+    return ts, r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--side", type=int, default=1000)
-    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="grid1000", choices=["grid1000", "rmat20"])
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--queries", type=int, default=256)
-    ap.add_argument("--delta", type=int, default=0, help="Delta of the tuned path (0 = tuned table / default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -52,70 +66,85 @@ def main():
         sys.exit(2)
 
     import pathfinding_gpu_amd as P
+    from pathfinding_gpu_amd import learned as L
 
     t0 = time.time()
-    g = P.grid_road(args.side, args.side, args.seed)
-    log(f"[bench] graph {g.name}: n={g.n} m={g.m} avgW={g.avg_w:.2f} built in {time.time() - t0:.1f}s")
-    K = args.queries
-    sv = P.GpuSolver(g, max_batch=max(1, K))
+    if args.workload == "grid1000":
+        g = P.grid_road(1000, 1000, 7)
+        src0 = 0
+    else:
+        g = P.rmat(20, 16, 11)
+        src0 = g.max_degree_vertex()
+    log(f"[bench] graph {g.name}: n={g.n} m={g.m} avgW={g.avg_w:.2f} avgDeg={g.avg_deg:.2f} ({time.time() - t0:.1f}s)")
+    K = args.batch
+    QA = args.queries if args.workload == "grid1000" else 0
+    sv = P.GpuSolver(g, max_batch=max(K, QA, 1))
     info = P.device_info(0)
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']} upload {sv.upload_ms():.1f} ms")
     auto = sv.auto_delta()
-
-    # ---- tuned device-resident path ----
-    delta = args.delta or 4 * auto
-    tuned = P.GpuOptions(algo=P.GpuAlgo.DeltaStep, drive=P.Drive.Persistent, delta=delta)
-    plain = P.GpuOptions(algo=P.GpuAlgo.DeltaStep, drive=P.Drive.Host, sync_every=8)   # reference defaults
-
-    def run_steps(opt, steps):
-        ts, st = [], None
-        for _ in range(steps):
-            a = time.perf_counter()
-            r = sv.sssp(0, opt, copy_back=False)
-            ts.append((time.perf_counter() - a) * 1e3)
-            st = r
-        return ts, st
-
-    log("[bench] warmup")
-    run_steps(tuned, args.warmup)
-    run_steps(plain, max(1, args.warmup))
-    log("[bench] timed: tuned path")
-    t_tuned, r_tuned = run_steps(tuned, args.steps)
-    reached, arcs = sv.last_reach()
-    log("[bench] timed: plain path")
-    t_plain, r_plain = run_steps(plain, args.steps)
-
-    ms_step = float(np.mean(t_tuned))
-    gteps = arcs / (ms_step * 1e-3) / 1e9
-    ms_plain = float(np.mean(t_plain))
-    gteps_plain = arcs / (ms_plain * 1e-3) / 1e9
-
-    # roofline of the step kernel (the persistent launch): algorithmic bytes per launch = 16 per reached
-    # vertex (row extent + own state) + 16 per scanned arc (col, w + target state), over its hipEvent duration
-    loop_ms = r_tuned.stats.loop_ms
-    bytes_launch = 16 * reached + 16 * arcs
-    achieved = bytes_launch / (loop_ms * 1e-3) / 1e9
-    b_alg = 8 * g.n + bytes_launch
-
-    # ---- A* batch ----
     rng = np.random.default_rng(2026)
-    srcs = rng.integers(0, g.n, K).astype(np.uint32)
-    dsts = rng.integers(0, g.n, K).astype(np.uint32)
-    aopt = P.GpuOptions(algo=P.GpuAlgo.DeltaStep, drive=P.Drive.Persistent, delta=delta,
-                        heuristic=P.Heuristic.FloorEuclid)
-    log("[bench] A* batch warmup")
-    sv.astar_batch(srcs, dsts, aopt, path_cap=8192)
-    a_ts = []
-    for _ in range(max(1, args.steps // 2)):
-        a = time.perf_counter()
-        res = sv.astar_batch(srcs, dsts, aopt, path_cap=8192)
-        a_ts.append((time.perf_counter() - a) * 1e3)
-    a_ms = float(np.mean(a_ts))
-    qps = K / (a_ms * 1e-3)
+    deg = np.diff(g.row_offsets)
+    cand = np.flatnonzero(deg > 0)
+    srcs = np.concatenate([[src0], rng.choice(cand[cand != src0], size=K - 1, replace=False)]).astype(np.uint32)
+
+    PERS = P.Drive.Persistent
+
+    def opt(delta, mapping=2, **kw):
+        return P.GpuOptions(drive=PERS, delta=int(delta), mapping=P.Mapping(mapping), **kw)
+
+    # ---- learned choices (network on the GPU, judged once against the Davidson default) ----
+    tuner = L.Tuner()
+
+    def tf_single(d, mp):
+        sv.sssp(src0, opt(d, mp), copy_back=False)
+        return min(sv.sssp(src0, opt(d, mp), copy_back=False).ms for _ in range(2))
+
+    def tf_batch(d, mp):
+        sv.multi_source(srcs, opt(d, mp), copy_back=False)
+        return min(sv.multi_source(srcs, opt(d, mp), copy_back=False).ms for _ in range(2))
+
+    log("[bench] choosing Delta")
+    pick1 = tuner.choose(sv, g, 1, tf_single)
+    pickK = tuner.choose(sv, g, K, tf_batch)
+    log(f"[bench] single: {pick1}")
+    log(f"[bench] batch:  {pickK}")
+
+    plain = opt(auto)
+    host_ref = P.GpuOptions(drive=P.Drive.Host, sync_every=8)          # reference defaults
+    learned1 = opt(pick1["delta"], pick1["mapping"])
+    learnedK = opt(pickK["delta"], pickK["mapping"])
+
+    # ---- warm-up, then timed steps: the learned batched run ----
+    for _ in range(args.warmup):
+        sv.multi_source(srcs, learnedK, copy_back=False)
+    log(f"[bench] timed: {args.steps} steps of K={K} batched SSSP")
+    t_step, r_step = timed(lambda: sv.multi_source(srcs, learnedK, copy_back=False), args.steps)
+    reached_k, arcs_k = sv.last_reach()
+    ms_step = float(np.mean(t_step))
+    gteps = arcs_k / (ms_step * 1e-3) / 1e9
+
+    # ---- plain and learned single query ----
+    reps = max(3, args.steps // 2)
+    sv.sssp(src0, plain, copy_back=False)
+    t_plain, r_plain = timed(lambda: sv.sssp(src0, plain, copy_back=False), reps)
+    reached1, arcs1 = sv.last_reach()
+    sv.sssp(src0, host_ref, copy_back=False)
+    t_host, r_host = timed(lambda: sv.sssp(src0, host_ref, copy_back=False), reps)
+    sv.sssp(src0, learned1, copy_back=False)
+    t_l1, r_l1 = timed(lambda: sv.sssp(src0, learned1, copy_back=False), reps)
+    ms_plain, ms_host, ms_l1 = float(np.mean(t_plain)), float(np.mean(t_host)), float(np.mean(t_l1))
+    gteps_plain = arcs1 / (ms_plain * 1e-3) / 1e9
+
+    # roofline of the dominant kernel: the persistent step kernel of the batched run (one launch per step).
+    # Algorithmic bytes per launch: 16 per reached (k, v) (row extent + own state) + 16 per scanned arc
+    # (col, w + target state) over all K queries; duration = its hipEvent span on the engine stream.
+    bytes_launch = 16 * reached_k + 16 * arcs_k
+    loop_ms = r_step.stats.loop_ms
+    achieved = bytes_launch / (loop_ms * 1e-3) / 1e9
 
     out = {
-        "metric": "GTEPS (SSSP, 1000x1000 grid road graph, tuned device-resident Delta-stepping) "
-                  "+ A* queries/s and latency, bit-exact",
+        "metric": "GTEPS, learned-Delta batched (K=64 SSSP per launch) vs plain default-Delta single query; "
+                  "+ A* queries/s and latency; bit-exact",
         "value": round(gteps, 4),
         "unit": "GTEPS",
         "n_gpus": 1,
@@ -127,57 +156,96 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": f"grid{args.side}x{args.side}_seed{args.seed}: sssp from 0 + {K} A* queries",
-                   "n": g.n, "m": g.m, "delta": delta, "auto_delta": auto, "drive": "persistent",
-                   "astar_heuristic": "floor_euclid"},
-        "plain_gteps": round(gteps_plain, 4),
-        "plain_ms": round(ms_plain, 4),
-        "speedup_vs_plain": round(ms_plain / ms_step, 3),
-        "plain_stats": {"iterations": r_plain.stats.iterations, "host_syncs": r_plain.stats.host_syncs,
-                        "launches": r_plain.stats.launches, "buckets": r_plain.stats.buckets},
-        "tuned_stats": {"iterations": r_tuned.stats.iterations, "steps": r_tuned.stats.steps,
-                        "buckets": r_tuned.stats.buckets, "edges_scanned": r_tuned.stats.edges_scanned,
-                        "atomics_per_update": round(r_tuned.stats.atomics_issued / max(1, r_tuned.stats.updates), 4),
-                        "event_ms": round(r_tuned.ms, 4), "loop_ms": round(loop_ms, 4)},
-        "astar": {"queries": K, "batch_ms": round(a_ms, 3), "queries_per_s": round(qps, 1),
-                  "p50_ms": round(float(np.percentile(a_ts, 50)), 3),
-                  "p99_ms": round(float(np.percentile(a_ts, 99)), 3)},
+        "config": {"workload": f"{g.name}: SSSP from {src0} (+{K - 1} random sources per batch)"
+                               + (f" + {QA} A* queries" if QA else ""),
+                   "n": g.n, "m": g.m, "batch": K, "auto_delta": auto,
+                   "learned_single": {"delta": pick1["delta"], "mapping": pick1["mapping"], "source": pick1["source"]},
+                   "learned_batch": {"delta": pickK["delta"], "mapping": pickK["mapping"], "source": pickK["source"]},
+                   "drive": "persistent"},
+        "plain_gteps": round(gteps_plain, 5),
+        "plain_ms": round(ms_plain, 3),
+        "speedup_vs_plain": round(gteps / gteps_plain, 3),
+        "host_driven_reference_style_ms": round(ms_host, 3),
+        "learned_single_ms": round(ms_l1, 3),
+        "learned_single_gteps": round(arcs1 / (ms_l1 * 1e-3) / 1e9, 5),
+        "learned_single_speedup": round(ms_plain / ms_l1, 3),
+        "step_stats": {"steps": r_step.stats.steps, "buckets": r_step.stats.buckets,
+                       "edges_scanned": r_step.stats.edges_scanned,
+                       "atomics_per_update": round(r_step.stats.atomics_issued / max(1, r_step.stats.updates), 4),
+                       "event_ms": round(r_step.ms, 3), "loop_ms": round(loop_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "bytes_per_launch": int(bytes_launch), "b_alg_per_sssp": int(b_alg)},
+                     "bytes_per_launch": int(bytes_launch), "kernel": "stepKernel (persistent, K-source batch)"},
     }
+
+    if QA:
+        pa = np.stack([rng.integers(0, g.n, QA), rng.integers(0, g.n, QA)], 1).astype(np.uint32)
+        pickA = L.DeltaModel.load().choose(g, QA) if tuner.model is not None else {"delta": auto, "mapping": 2}
+        aopt = opt(pickA["delta"], pickA["mapping"], heuristic=P.Heuristic.FloorEuclid)
+        sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192)
+        a_ts, _ = timed(lambda: sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192), max(2, args.steps // 3))
+        a_ms = float(np.mean(a_ts))
+        popt = opt(auto, heuristic=P.Heuristic.FloorEuclid)
+        nq = 8
+        sv.astar(int(pa[0, 0]), int(pa[0, 1]), popt, path_cap=8192)
+        p_ts, _ = timed(lambda: [sv.astar(int(pa[i, 0]), int(pa[i, 1]), popt, path_cap=8192) for i in range(nq)], 1)
+        out["astar"] = {"queries": QA, "batch_ms": round(a_ms, 3), "queries_per_s": round(QA / (a_ms * 1e-3), 1),
+                        "latency_ms_p50": round(float(np.percentile(a_ts, 50)), 3),
+                        "latency_ms_p99": round(float(np.percentile(a_ts, 99)), 3),
+                        "delta": pickA["delta"], "mapping": pickA["mapping"],
+                        "plain_one_at_a_time_queries_per_s": round(nq / (p_ts[0] * 1e-3), 1),
+                        "note": "all queries of a batch complete together: latency = batch time"}
 
     if not args.no_cpu:
         log("[bench] CPU baseline")
+        import ctypes as C
+
         import oracle
-        kind = "reference" if oracle.ref() is not None else "port"
+        R = oracle.ref()
+        kind = "reference" if R is not None else "port"
         row, col, w = g.row_offsets, g.col_indices, g.weights
+        u32p = C.POINTER(C.c_uint32)
         if kind == "reference":
-            import ctypes as C
-            R = oracle.ref()
-            xs = None
-            gh = R.ref_build_csr(g.n, g.m, *(a.ctypes.data_as(C.POINTER(C.c_uint32))
-                                              for a in (np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(row)),
-                                                        col, w)), xs, xs)
+            srcv = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(row))
+            gh = R.ref_build_csr(g.n, g.m, srcv.ctypes.data_as(u32p), col.ctypes.data_as(u32p), w.ctypes.data_as(u32p),
+                                 None, None)
             dist = np.empty(g.n, np.uint32)
             par = np.empty(g.n, np.uint32)
-            reps, tot = 0, 0.0
-            while tot < args.cpu_seconds * 1e3 and reps < 200:
-                tot += R.ref_dijkstra(gh, 0, dist.ctypes.data_as(C.POINTER(C.c_uint32)),
-                                      par.ctypes.data_as(C.POINTER(C.c_uint32)), None)
-                reps += 1
+            reps_c, tot = 0, 0.0
+            while tot < args.cpu_seconds * 1e3 * 0.5 and reps_c < 100:
+                tot += R.ref_dijkstra(gh, int(src0), dist.ctypes.data_as(u32p), par.ctypes.data_as(u32p), None)
+                reps_c += 1
+            kk = min(K, 64)
+            md = np.empty(kk * g.n, np.uint32)
+            e = C.c_uint64()
+            thr = C.c_int()
+            ms_omp = R.ref_multi_source(gh, srcs[:kk].ctypes.data_as(u32p), kk, 0, md.ctypes.data_as(u32p), C.byref(e),
+                                        C.byref(thr))
             R.ref_free(gh)
+            threads = thr.value
         else:
-            reps = max(1, int(args.cpu_seconds * 1e3 / max(1.0, oracle.time_dijkstra(row, col, w, 0, 1))))
-            tot = oracle.time_dijkstra(row, col, w, 0, reps)
-            dist = oracle.dijkstra(row, col, w, 0)[0]
-        cpu_ms = tot / reps
-        chk = sv.sssp(0, tuned)
-        out["bitexact_dist_vs_cpuDijkstra"] = bool(np.array_equal(chk.dist, dist))
-        out["cpu_baseline"] = {"value": round(arcs / (cpu_ms * 1e-3) / 1e9, 5), "unit": "GTEPS", "cores": 1,
+            reps_c = max(1, int(args.cpu_seconds * 500 / max(1.0, oracle.time_dijkstra(row, col, w, int(src0), 1))))
+            tot = oracle.time_dijkstra(row, col, w, int(src0), reps_c)
+            dist = oracle.dijkstra(row, col, w, int(src0))[0]
+            kk = min(K, 64)
+            a = time.perf_counter()
+            md, _, threads = oracle.multi_source(row, col, w, srcs[:kk], 0)
+            ms_omp = (time.perf_counter() - a) * 1e3
+            md = md.ravel()
+        cpu_ms = tot / reps_c
+        gpu1 = sv.sssp(int(src0), learned1)
+        gpuK = sv.multi_source(srcs[:kk] if kk == K else srcs, learnedK)
+        out["bitexact_vs_reference_cpu"] = {
+            "single_dist": bool(np.array_equal(gpu1.dist, dist)),
+            "batch_dist": bool(np.array_equal(gpuK.dist[:kk].ravel(), md)),
+        }
+        out["cpu_baseline"] = {"value": round(arcs1 / (cpu_ms * 1e-3) / 1e9, 5), "unit": "GTEPS", "cores": 1,
                                "kind": kind, "ms_per_sssp": round(cpu_ms, 3),
-                               "sample": f"cpuDijkstra from vertex 0 on the same graph, {reps} runs",
-                               "host_threads": os.cpu_count()}
+                               "sample": f"cpuDijkstra from vertex {src0} on the same graph, {reps_c} runs"}
+        out["cpu_multisource_omp"] = {"value": round(arcs1 * kk / (ms_omp * 1e-3) / 1e9, 5), "unit": "GTEPS",
+                                      "cores": threads, "kind": kind, "ms": round(ms_omp, 2),
+                                      "sample": f"cpuMultiSource over the first {kk} batch sources"}
+        out["host"] = {"cpu_threads": os.cpu_count()}
     sv.close()
     print(json.dumps(out))
 

@@ -154,6 +154,28 @@ int pf_multi_seed(pf_solver* s, const uint32_t* seeds, uint32_t nseeds, const pf
  * all k slots of the last run. */
 int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs);
 
+/* ---- serving: continuous batching (no reference counterpart) --------------
+ * The solver's max_batch slots stay resident; queries are admitted into idle
+ * slots between epochs of one persistent kernel.  An admitted query enters the
+ * engine's far pile with bound 0, so it joins at the next bucket transition
+ * while the other slots keep running; a slot is done when its far-pile minimum
+ * is empty.  Per-slot results are read while other slots continue.
+ *   pf_serve_begin  : all slots idle (astar != 0: slots carry targets)
+ *   pf_serve_admit  : reset + seed `count` idle slots (enqueued on the stream)
+ *   pf_serve_run    : one persistent epoch of at most epoch_steps steps
+ *                     (0 = until idle); done[K] = 1 for idle/finished slots
+ *   pf_serve_*_result: the answer of finished slots
+ * Calls on one solver must not overlap; other run entry points fail with
+ * PF_ERR_STATE until pf_serve_end. */
+int pf_serve_begin(pf_solver* s, const pf_options* o, int astar);
+int pf_serve_admit(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts, uint32_t count);
+int pf_serve_run(pf_solver* s, uint32_t epoch_steps, uint32_t* done, uint32_t* idle, pf_stats* st);
+int pf_serve_astar_results(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts,
+                           uint32_t count, uint32_t* target_dists, uint32_t* paths, uint32_t path_cap,
+                           uint32_t* path_lens);
+int pf_serve_sssp_result(pf_solver* s, uint32_t slot, uint32_t* dist, uint32_t* parent);
+int pf_serve_end(pf_solver* s);
+
 /* Test hooks: the engine's own device computations for parity tests. */
 int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end); /* n entries */
 int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h); /* n entries */

@@ -79,6 +79,12 @@ ENGINE_SYMBOLS = {
     "pf_debug_heuristic": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, u32p]),
     "pf_debug_exclusive_scan": (C.c_int, [u32p, u32p, C.c_uint32, u32p]),
     "pf_debug_sorted_rows": (C.c_int, [C.c_void_p, u32p, u32p]),
+    "pf_serve_begin": (C.c_int, [C.c_void_p, C.POINTER(PfOptions), C.c_int]),
+    "pf_serve_admit": (C.c_int, [C.c_void_p, u32p, u32p, u32p, C.c_uint32]),
+    "pf_serve_run": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, C.POINTER(PfStats)]),
+    "pf_serve_astar_results": (C.c_int, [C.c_void_p, u32p, u32p, u32p, C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
+    "pf_serve_sssp_result": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p]),
+    "pf_serve_end": (C.c_int, [C.c_void_p]),
     "pf_mlp_last_error": (C.c_char_p, []),
     "pf_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pf_mlp_destroy": (None, [C.c_void_p]),

@@ -123,6 +123,14 @@ struct pf_solver {
     double wallClockHz = 1e8;
     u32 lastK = 0;
     std::vector<GraphCacheEntry> graphs;
+    // serving
+    bool serving = false;
+    Run serveRun{};
+    StepFn serveFn = nullptr;
+    u32 serveGrid = 0, serveBs = 256, serveShm = 0;
+    u32* d_admit = nullptr;   // [3*K]: slots, sources, targets of one admission
+    u32* h_admit = nullptr;   // pinned mirror
+    u32* h_done = nullptr;    // pinned [K]
 
     template <class T>
     int dalloc(T** p, u64 count) {
@@ -276,9 +284,37 @@ int ensureFlags(pf_solver* s, u32 tileItems) {
     return PF_OK;
 }
 
+Run baseRun(const pf_solver* s) {
+    Run run{};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.state = s->d_state;
+    run.ctl = s->d_ctl;
+    run.stats = s->d_stats;
+    for (int i = 0; i < 2; ++i) {
+        run.nearQ[i] = s->d_near[i];
+        run.nearMask[i] = s->d_nearMask[i];
+        run.farQ[i] = s->d_far[i];
+        run.act[i] = s->d_act[i];
+    }
+    run.farMask = s->d_farMask;
+    run.rQ = s->d_r;
+    run.rMask = s->d_rMask;
+    run.tileSums = s->d_tileSums;
+    run.bound = s->d_bound;
+    run.newBound = s->d_newBound;
+    run.minKey = s->d_minKey;
+    run.slotDone = s->d_slotDone;
+    run.hScale = s->hScale;
+    run.invN = 1.0 / (double)s->n;
+    run.bar = s->d_bar;
+    run.spinTicks = (unsigned long long)(2.0 * s->wallClockHz);
+    return run;
+}
+
 int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     int rc = setDev(s);
     if (rc) return rc;
+    if (s->serving) return fail(PF_ERR_STATE, "solver is serving: call pf_serve_end first");
     pf_options o = spec.o;
     sanitize(o);
     if (spec.k == 0 || spec.k > s->K) return fail(PF_ERR_INVALID, "invalid batch size");
@@ -754,6 +790,9 @@ void pf_solver_destroy(pf_solver* s) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (s->h_ctl) hipHostFree(s->h_ctl);
+    if (s->h_admit) hipHostFree(s->h_admit);
+    if (s->h_done) hipHostFree(s->h_done);
+    if (s->d_admit) hipFree(s->d_admit);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->evLoop) hipEventDestroy(s->evLoop);
@@ -842,6 +881,180 @@ int pf_astar(pf_solver* s, uint32_t src, uint32_t dst, const pf_options* o, uint
     if (src >= s->n) return fail(PF_ERR_INVALID, "source out of range");
     if (dst >= s->n) return fail(PF_ERR_INVALID, "target out of range");
     return pf_astar_batch(s, &src, &dst, 1, o, target_dist, path, path_cap, path_len, st);
+}
+
+// ---- serving: continuous batching over the solver's K slots ----------------
+int pf_serve_begin(pf_solver* s, const pf_options* opt, int astar) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!opt) return fail(PF_ERR_INVALID, "null options");
+    pf_options o = *opt;
+    sanitize(o);
+    if (o.algo != PF_ALGO_DELTA) return fail(PF_ERR_INVALID, "serving runs Delta-stepping");
+    const u32 delta = o.delta ? o.delta : autoDelta(s);
+    rc = ensureLightEnd(s, delta);
+    if (rc) return rc;
+    u32 hk = HK_NONE;
+    if (astar) {
+        if (o.heuristic == PF_H_FLOOR_EUCLID) {
+            if (!s->intCoords) return fail(PF_ERR_INVALID, "FLOOR_EUCLID heuristic needs integer coordinates");
+            hk = HK_FLOOR_EUCLID;
+        } else if (o.heuristic == PF_H_SCALED && s->hasCoords && s->hScale > 0.0) {
+            hk = HK_SCALED;
+        }
+    }
+    if (!s->d_admit) {
+        if ((rc = s->dalloc(&s->d_admit, 3ull * s->K))) return rc;
+        PF_HIP(hipHostMalloc((void**)&s->h_admit, 3ull * s->K * 4, hipHostMallocDefault));
+        PF_HIP(hipHostMalloc((void**)&s->h_done, (size_t)s->K * 4, hipHostMallocDefault));
+    }
+    const u64 Kn = (u64)s->K * s->n;
+    Run run = baseRun(s);
+    run.flags = nullptr;
+    run.target = astar ? s->d_target : nullptr;
+    run.K = s->K;
+    run.delta = delta;
+    run.hkind = hk;
+    run.waveCutoff = o.wave_cutoff ? o.wave_cutoff : 64u;
+    run.blockCutoff = o.block_cutoff ? o.block_cutoff : (u32)o.block_size;
+    run.tileItems = (u32)o.block_size * 16;
+    run.numTiles = divUp(Kn, run.tileItems);
+    run.total = (u32)Kn;
+    const u32 bs = (u32)o.block_size;
+    const u32 shm = (bs / 64) * 64 * sizeof(GatherSlot);
+    const StepFn fn = stepTable[astar ? 1 : 0][o.atomic][o.mapping];
+    int nb = 0;
+    PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, (int)bs, shm));
+    const u32 maxCo = (u32)std::max(1, nb) * (u32)s->cuCount;
+    const u32 want = o.grid_blocks ? o.grid_blocks : 2u * (u32)s->cuCount;
+    s->serveGrid = std::max(1u, std::min(want, maxCo));
+    s->serveBs = bs;
+    s->serveShm = shm;
+    s->serveFn = fn;
+    s->serveRun = run;
+    const u32 initGrid = std::min<u32>(divUp(Kn, 256), 8192u);
+    hipLaunchKernelGGL(initKernel, dim3(initGrid), dim3(256), 0, s->s, s->d_state, Kn, s->d_nearMask[0],
+                       s->d_nearMask[1], s->d_farMask, s->d_rMask, maskWords(Kn), nullptr, 0ull, nullptr, nullptr, 0u);
+    hipLaunchKernelGGL(serveInitKernel, dim3(divUp(s->K, 256)), dim3(256), 0, s->s, run);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipStreamSynchronize(s->s));
+    s->serving = true;
+    s->lastK = s->K;
+    return PF_OK;
+}
+
+int pf_serve_admit(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts, uint32_t count) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->serving) return fail(PF_ERR_STATE, "pf_serve_begin first");
+    if (count == 0) return PF_OK;
+    if (count > s->K) return fail(PF_ERR_INVALID, "more admissions than slots");
+    for (u32 i = 0; i < count; ++i) {
+        if (slots[i] >= s->K) return fail(PF_ERR_INVALID, "slot out of range");
+        if (srcs[i] >= s->n || (dsts && dsts[i] >= s->n)) return fail(PF_ERR_INVALID, "source/target out of range");
+        for (u32 j = 0; j < i; ++j)
+            if (slots[j] == slots[i]) return fail(PF_ERR_INVALID, "slot admitted twice");
+    }
+    u32* h = s->h_admit;
+    std::memcpy(h, slots, (size_t)count * 4);
+    std::memcpy(h + s->K, srcs, (size_t)count * 4);
+    if (dsts) std::memcpy(h + 2 * s->K, dsts, (size_t)count * 4);
+    else std::memset(h + 2 * s->K, 0, (size_t)count * 4);
+    PF_HIP(hipMemcpyAsync(s->d_admit, h, 3ull * s->K * 4, hipMemcpyHostToDevice, s->s));
+    const u64 total = (u64)count * s->n;
+    hipLaunchKernelGGL(slotInitKernel, dim3(std::min<u32>(divUp(total, 256), 8192u)), dim3(256), 0, s->s, s->d_state,
+                       s->n, s->d_admit, count);
+    hipLaunchKernelGGL(admitKernel, dim3(1), dim3(256), 0, s->s, s->serveRun, s->d_admit, s->d_admit + s->K,
+                       s->d_admit + 2 * s->K, count);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_serve_run(pf_solver* s, uint32_t epoch_steps, uint32_t* done, uint32_t* idle, pf_stats* st) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->serving) return fail(PF_ERR_STATE, "pf_serve_begin first");
+    hipLaunchKernelGGL(epochKernel, dim3(1), dim3(1), 0, s->s, s->serveRun, epoch_steps);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipEventRecord(s->ev0, s->s));
+    rc = launchStep(s, s->serveFn, s->serveGrid, s->serveBs, s->serveShm, s->serveRun, 1);
+    if (rc) return rc;
+    PF_HIP(hipEventRecord(s->ev1, s->s));
+    PF_HIP(hipMemcpyAsync(s->h_ctl, s->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipMemcpyAsync(s->h_done, s->d_slotDone, (size_t)s->K * 4, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    const Ctl& hc = *s->h_ctl;
+    if (hc.error & E_BARRIER) return fail(PF_ERR_HIP, "grid barrier wait exceeded its spin limit");
+    if (hc.error & E_NOT_CONVERGED) return fail(PF_ERR_NOT_CONVERGED, "relaxation did not converge");
+    if (done) std::memcpy(done, s->h_done, (size_t)s->K * 4);
+    if (idle) *idle = hc.phase == PH_DONE ? 1u : 0u;
+    if (st) {
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        std::memset(st, 0, sizeof(*st));
+        st->ms = ms;
+        st->loop_ms = ms;
+        st->steps = hc.steps;
+        st->iterations = hc.iters;
+        st->buckets = hc.buckets;
+        st->launches = 1;
+        st->host_syncs = 1;
+        st->delta = s->serveRun.delta;
+    }
+    return PF_OK;
+}
+
+int pf_serve_astar_results(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts,
+                           uint32_t count, uint32_t* target_dists, uint32_t* paths, uint32_t path_cap,
+                           uint32_t* path_lens) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->serving) return fail(PF_ERR_STATE, "pf_serve_begin first");
+    if (count == 0) return PF_OK;
+    if (count > s->K) return fail(PF_ERR_INVALID, "too many slots");
+    for (u32 i = 0; i < count; ++i)
+        if (slots[i] >= s->K || srcs[i] >= s->n || dsts[i] >= s->n) return fail(PF_ERR_INVALID, "slot/vertex out of range");
+    const u64 need = (u64)count * path_cap;
+    if (need > s->pathCap) {
+        if (s->d_paths) hipFree(s->d_paths);
+        s->d_paths = nullptr;
+        if ((rc = s->dalloc(&s->d_paths, need))) return rc;
+        s->pathCap = need;
+    }
+    u32* h = s->h_admit;
+    std::memcpy(h, slots, (size_t)count * 4);
+    std::memcpy(h + s->K, srcs, (size_t)count * 4);
+    std::memcpy(h + 2 * s->K, dsts, (size_t)count * 4);
+    PF_HIP(hipMemcpyAsync(s->d_admit, h, 3ull * s->K * 4, hipMemcpyHostToDevice, s->s));
+    hipLaunchKernelGGL(chaseSlotsKernel, dim3(divUp(count, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_admit,
+                       s->d_admit + s->K, s->d_admit + 2 * s->K, count, s->d_paths, path_cap, s->d_lens, s->d_tdist);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpyAsync(target_dists, s->d_tdist, (size_t)count * 4, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipMemcpyAsync(path_lens, s->d_lens, (size_t)count * 4, hipMemcpyDeviceToHost, s->s));
+    if (paths) PF_HIP(hipMemcpyAsync(paths, s->d_paths, need * 4, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    return PF_OK;
+}
+
+int pf_serve_sssp_result(pf_solver* s, uint32_t slot, uint32_t* dist, uint32_t* parent) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->serving) return fail(PF_ERR_STATE, "pf_serve_begin first");
+    if (slot >= s->K) return fail(PF_ERR_INVALID, "slot out of range");
+    std::vector<u64> hst(s->n);
+    PF_HIP(hipMemcpyAsync(hst.data(), s->d_state + (u64)slot * s->n, (size_t)s->n * 8, hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    for (u32 v = 0; v < s->n; ++v) {
+        if (dist) dist[v] = (u32)(hst[v] >> 32);
+        if (parent) parent[v] = (u32)hst[v];
+    }
+    return PF_OK;
+}
+
+int pf_serve_end(pf_solver* s) {
+    if (!s) return fail(PF_ERR_INVALID, "null solver");
+    s->serving = false;
+    return PF_OK;
 }
 
 int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs) {

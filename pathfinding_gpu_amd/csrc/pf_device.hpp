@@ -916,6 +916,109 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
     }
 }
 
+// ---------------------------------------------------------------------------
+// serving: admission of new queries into idle slots of a resident run
+// ---------------------------------------------------------------------------
+// Resets the state rows of the admitted slots (grid-stride over count * n).
+__global__ void slotInitKernel(u64* state, u32 n, const u32* slots, u32 count) {
+    const u64 total = (u64)count * n;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
+        const u32 j = (u32)(i / n);
+        state[(u64)slots[j] * n + (i - (u64)j * n)] = STATE_INIT;
+    }
+}
+
+// One block.  Seeds each admitted slot at its source and puts the seed into the
+// current far pile with bound 0, so the next far-pile minimum opens the slot's
+// first bucket; the slots of running queries are untouched.  From an idle
+// engine (phase DONE) the next step is that minimum.
+__global__ void admitKernel(Run r, const u32* slots, const u32* srcs, const u32* dsts, u32 count) {
+    Ctl* c = r.ctl;
+    __shared__ u32 s_wasDone;
+    if (threadIdx.x == 0) s_wasDone = c->phase == PH_DONE;
+    __syncthreads();
+    if (s_wasDone) {   // every slot's far minimum is recomputed from scratch
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.minKey[k] = NOKEY;
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < count; i += blockDim.x) {
+        const u32 k = slots[i];
+        const u32 gi = k * r.g.n + srcs[i];
+        r.state[gi] = pack(0u, NOPAR);
+        if (r.target) const_cast<u32*>(r.target)[k] = dsts[i];
+        r.bound[k] = 0ull;
+        r.newBound[k] = 0ull;
+        r.minKey[k] = NOKEY;
+        r.slotDone[k] = 0u;
+        const u32 fcur = c->fcur;
+        atomicOr(r.farMask + (gi >> 5), 1u << (gi & 31u));
+        r.farQ[fcur][atomicAdd(&c->cnt[C_FAR0 + fcur], 1u)] = gi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_wasDone) {
+        c->phase = PH_FARMIN;
+        c->error = 0u;
+    }
+}
+
+// Serving set-up: every slot idle (done), engine phase DONE, no step limit.
+__global__ void serveInitKernel(Run r) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < r.K; k += gridDim.x * blockDim.x) {
+        r.bound[k] = 0ull;
+        r.newBound[k] = 0ull;
+        r.minKey[k] = NOKEY;
+        r.slotDone[k] = 1u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Ctl* c = r.ctl;
+        c->phase = PH_DONE;
+        c->in = 0; c->fcur = 0; c->cur = 0;
+        for (int j = 0; j < C_COUNT; ++j) c->cnt[j] = 0;
+        c->arrive = 0; c->gen = 0;
+        c->steps = 0; c->iters = 0; c->buckets = 0; c->error = 0;
+        c->scanTotal = 0; c->epochEnd = 0; c->pause = 0; c->maxSteps = 0xFFFFFFFFu;
+        r.bar[0] = 0ull;
+        r.bar[BAR_GEN] = 0ull;
+        r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
+    }
+}
+
+// Next epoch: run at most `steps` steps from now (0 = until idle).
+__global__ void epochKernel(Run r, u32 steps) {
+    Ctl* c = r.ctl;
+    c->epochEnd = steps ? c->steps + steps : 0u;
+    c->pause = 0u;
+    r.bar[0] = 0ull;
+    r.bar[BAR_GEN] = 0ull;
+}
+
+// Paths of selected slots, followed on the device (reconstructPath,
+// src/cpu_sssp.cpp:124-135): one thread per listed slot.
+__global__ void chaseSlotsKernel(const u64* state, u32 n, const u32* slots, const u32* srcs, const u32* targets,
+                                 u32 count, u32* paths, u32 cap, u32* lens, u32* tdist) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const u64* st = state + (u64)slots[i] * n;
+    const u32 s = srcs[i], t = targets[i];
+    u32* p = paths + (u64)i * cap;
+    const u32 d = distOf(st[t]);
+    tdist[i] = d;
+    u32 len = 0;
+    if (d != INF) {
+        u32 v = t;
+        bool ok = false;
+        for (u32 steps = 0; steps <= n && len < cap; ++steps) {
+            p[len++] = v;
+            if (v == s) { ok = true; break; }
+            v = (u32)st[v];
+            if (v == NOPAR || v >= n) break;
+        }
+        if (!ok) len = 0;
+        for (u32 j = 0; j < len / 2; ++j) { const u32 x = p[j]; p[j] = p[len - 1 - j]; p[len - 1 - j] = x; }
+    }
+    lens[i] = len;
+}
+
 // lightEndKernel (src/gpu_kernels.cuh:361-371).
 __global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, u32* lightEnd) {
     const u32 u = blockIdx.x * blockDim.x + threadIdx.x;

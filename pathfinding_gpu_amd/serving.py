@@ -1,0 +1,206 @@
+"""Continuous-batching query server over resident graphs (no reference counterpart).
+
+Each resident graph has one solver in serving mode (pf_serve_*): its K slots hold
+the per-query state (K*n*8 bytes of packed (dist, parent) plus queues) in HBM.
+Requests arrive as a Poisson stream; whenever slots are free, arrived requests
+are admitted between epochs of the persistent kernel, while the queries already
+in flight keep running in the same kernel.  Finished slots are read back (A*:
+target distance and path followed on the device; SSSP: distances) and freed.
+Latency of a request = completion time - arrival time on the host clock.
+
+The number of slots of a graph is fixed when the server starts, from the free
+HBM (`slots_for_memory`), which is the memory-based admission: a request waits
+until a slot -- i.e. K*n*8 bytes of state -- is free.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _native as N
+from .graph import CSRGraph
+from .solver import GpuOptions, GpuSolver, INF_DIST
+
+BYTES_PER_SLOT_VERTEX = 8 + 4 * 5 + 4 * 4 / 32.0   # state + 2 near + 2 far + R queues + 4 bitmasks
+
+
+def slots_for_memory(n: int, free_bytes: int, fraction: float = 0.5, cap: int = 1024) -> int:
+    """Slots that fit in `fraction` of the free HBM for a graph of n vertices."""
+    per = BYTES_PER_SLOT_VERTEX * n
+    k = int(free_bytes * fraction // per)
+    return max(1, min(cap, k, (0xFFFFFFFF - 1) // max(n, 1)))
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds from the start of the run
+    graph: str              # key of the resident graph
+    src: int
+    dst: int = -1           # A* target (-1: full SSSP)
+    admitted: float = -1.0
+    done: float = -1.0
+    dist: Optional[int] = None
+    path: Optional[np.ndarray] = None
+    dist_all: Optional[np.ndarray] = None
+
+
+class SlotEngine:
+    """One resident graph in serving mode."""
+
+    def __init__(self, g: CSRGraph, slots: int, opt: GpuOptions, astar: bool, keep_sssp: bool = True,
+                 path_cap: int = 16384):
+        self.g = g
+        self.astar = astar
+        self.K = slots
+        self.sv = GpuSolver(g, max_batch=slots)
+        self._lib = self.sv._lib
+        self.opt = opt
+        N.check(self._lib.pf_serve_begin(self.sv._h, C.byref(opt.to_c()), 1 if astar else 0))
+        self.free: List[int] = list(range(slots))[::-1]
+        self.busy: Dict[int, Request] = {}
+        self.keep_sssp = keep_sssp
+        self.path_cap = path_cap
+        self.epochs = 0
+        self.gpu_ms = 0.0
+
+    def close(self):
+        if self.sv is not None:
+            self._lib.pf_serve_end(self.sv._h)
+            self.sv.close()
+            self.sv = None
+
+    def admit(self, reqs: List[Request], now: float) -> None:
+        if not reqs:
+            return
+        slots = np.array([self.free.pop() for _ in reqs], np.uint32)
+        srcs = np.array([r.src for r in reqs], np.uint32)
+        dsts = np.array([max(r.dst, 0) for r in reqs], np.uint32)
+        N.check(self._lib.pf_serve_admit(self.sv._h, N.ptr(slots), N.ptr(srcs), N.ptr(dsts) if self.astar else None,
+                                         len(reqs)))
+        for s, r in zip(slots, reqs):
+            r.admitted = now
+            self.busy[int(s)] = r
+
+    def epoch(self, steps: int) -> List[Request]:
+        """Runs one epoch; returns the requests that finished in it (results filled in)."""
+        done = np.empty(self.K, np.uint32)
+        idle = C.c_uint32()
+        st = N.PfStats()
+        N.check(self._lib.pf_serve_run(self.sv._h, steps, N.ptr(done), C.byref(idle), C.byref(st)))
+        self.epochs += 1
+        self.gpu_ms += st.ms
+        fin = [s for s in self.busy if done[s]]
+        out = []
+        if fin:
+            fin_a = np.array(fin, np.uint32)
+            if self.astar:
+                reqs = [self.busy[s] for s in fin]
+                srcs = np.array([r.src for r in reqs], np.uint32)
+                dsts = np.array([r.dst for r in reqs], np.uint32)
+                td = np.empty(len(fin), np.uint32)
+                lens = np.empty(len(fin), np.uint32)
+                paths = np.empty(len(fin) * self.path_cap, np.uint32)
+                N.check(self._lib.pf_serve_astar_results(self.sv._h, N.ptr(fin_a), N.ptr(srcs), N.ptr(dsts), len(fin),
+                                                         N.ptr(td), N.ptr(paths), self.path_cap, N.ptr(lens)))
+                for i, r in enumerate(reqs):
+                    r.dist = int(td[i])
+                    r.path = paths[i * self.path_cap: i * self.path_cap + int(lens[i])].copy()
+            else:
+                for s in fin:
+                    r = self.busy[s]
+                    if self.keep_sssp:
+                        d = np.empty(self.g.n, np.uint32)
+                        N.check(self._lib.pf_serve_sssp_result(self.sv._h, s, N.ptr(d), None))
+                        r.dist_all = d
+            for s in fin:
+                out.append(self.busy.pop(s))
+                self.free.append(s)
+        return out
+
+
+@dataclass
+class ServeReport:
+    requests: int
+    wall_s: float
+    qps: float
+    latency_ms: Dict[str, float]
+    per_graph: Dict[str, dict] = field(default_factory=dict)
+
+
+class Server:
+    """Continuous batching over several resident graphs."""
+
+    def __init__(self, engines: Dict[str, SlotEngine], epoch_steps: int = 64):
+        self.engines = engines
+        self.epoch_steps = epoch_steps
+
+    def run(self, requests: List[Request], time_limit_s: float = 120.0, log=None) -> ServeReport:
+        pending = {k: deque() for k in self.engines}
+        stream = deque(sorted(requests, key=lambda r: r.arrival))
+        finished: List[Request] = []
+        t0 = time.perf_counter()
+        last_log = 0.0
+        while stream or any(pending[k] for k in pending) or any(e.busy for e in self.engines.values()):
+            now = time.perf_counter() - t0
+            if now > time_limit_s:
+                raise TimeoutError(f"serving did not finish within {time_limit_s}s")
+            while stream and stream[0].arrival <= now:
+                r = stream.popleft()
+                pending[r.graph].append(r)
+            worked = False
+            for k, e in self.engines.items():
+                q = pending[k]
+                take = []
+                while q and e.free and len(take) < len(e.free):
+                    take.append(q.popleft())
+                e.admit(take, now)
+                if e.busy:
+                    for r in e.epoch(self.epoch_steps):
+                        r.done = time.perf_counter() - t0
+                        finished.append(r)
+                    worked = True
+            if not worked and stream:
+                wait = stream[0].arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.002))
+            if log and now - last_log > 1.0:
+                last_log = now
+                log(f"[serve] t={now:.1f}s finished={len(finished)} in-flight="
+                    f"{sum(len(e.busy) for e in self.engines.values())} queued={sum(len(q) for q in pending.values())}")
+        wall = time.perf_counter() - t0
+        lat = np.array([(r.done - r.arrival) * 1e3 for r in finished])
+        rep = ServeReport(len(finished), wall, len(finished) / wall,
+                          {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
+                           "p99": float(np.percentile(lat, 99)), "max": float(lat.max())})
+        for k, e in self.engines.items():
+            lk = np.array([(r.done - r.arrival) * 1e3 for r in finished if r.graph == k])
+            rep.per_graph[k] = {"slots": e.K, "epochs": e.epochs, "gpu_ms": round(e.gpu_ms, 2), "requests": int(lk.size),
+                                "p50_ms": float(np.percentile(lk, 50)) if lk.size else None,
+                                "p99_ms": float(np.percentile(lk, 99)) if lk.size else None}
+        return rep
+
+
+def poisson_requests(rate_qps: float, count: int, mix: Dict[str, float], sizes: Dict[str, int],
+                     astar_graphs: Dict[str, bool], seed: int = 0) -> List[Request]:
+    """`count` requests with exponential inter-arrival times (rate_qps), graph chosen by `mix`
+    probabilities, uniform random endpoints."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_qps, count)
+    t = np.cumsum(gaps) - gaps[0]
+    keys = list(mix)
+    p = np.array([mix[k] for k in keys], np.float64)
+    which = rng.choice(len(keys), size=count, p=p / p.sum())
+    out = []
+    for i in range(count):
+        k = keys[which[i]]
+        n = sizes[k]
+        s = int(rng.integers(0, n))
+        d = int(rng.integers(0, n)) if astar_graphs[k] else -1
+        out.append(Request(i, float(t[i]), k, s, d))
+    return out

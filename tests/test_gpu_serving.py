@@ -1,0 +1,87 @@
+"""GPU: continuous batching gives every request the same answer alone and in the mix, bit-exact to the oracle."""
+import numpy as np
+import pytest
+
+import pathfinding_gpu_amd as P
+from pathfinding_gpu_amd.serving import Request, Server, SlotEngine, poisson_requests
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    return {"grid": P.grid_road(60, 60, 3), "rmat": P.rmat(11, 16, 5)}
+
+
+def _engines(graphs, slots=(6, 3)):
+    o = P.GpuOptions(drive=P.Drive.Persistent, heuristic=P.Heuristic.FloorEuclid)
+    return {"grid": SlotEngine(graphs["grid"], slots[0], o, astar=True),
+            "rmat": SlotEngine(graphs["rmat"], slots[1], P.GpuOptions(drive=P.Drive.Persistent), astar=False)}
+
+
+def _check_oracle(orc, graphs, r):
+    g = graphs[r.graph]
+    if r.dst >= 0:
+        d, path, _, _ = orc.astar(g.row_offsets, g.col_indices, g.weights, g.x, g.y, g.h_scale, orc.H_FLOOR_EUCLID,
+                                  r.src, r.dst, True)
+        assert r.dist == d, r
+        assert np.array_equal(r.path, path), r
+    else:
+        d, _, _, _, _ = orc.dijkstra(g.row_offsets, g.col_indices, g.weights, r.src)
+        assert np.array_equal(r.dist_all, d), r
+
+
+def test_mix_equals_alone_and_oracle(orc, graphs):
+    reqs = poisson_requests(3000.0, 40, {"grid": 0.75, "rmat": 0.25}, {k: g.n for k, g in graphs.items()},
+                            {"grid": True, "rmat": False}, seed=11)
+    eng = _engines(graphs)
+    try:
+        rep = Server(eng, epoch_steps=3).run(reqs, time_limit_s=60)
+    finally:
+        for e in eng.values():
+            e.close()
+    assert rep.requests == len(reqs)
+    # admissions overlapped running queries: some request was admitted while another was in flight
+    overl = any(a.admitted > b.admitted and a.admitted < b.done and a.graph == b.graph for a in reqs for b in reqs)
+    assert overl
+    for r in reqs:
+        _check_oracle(orc, graphs, r)
+    # the same requests alone, one server run each
+    for r in reqs[:12]:
+        eng = _engines(graphs, slots=(1, 1))
+        alone = Request(r.rid, 0.0, r.graph, r.src, r.dst)
+        try:
+            Server(eng, epoch_steps=7).run([alone], time_limit_s=30)
+        finally:
+            for e in eng.values():
+                e.close()
+        if r.dst >= 0:
+            assert alone.dist == r.dist and np.array_equal(alone.path, r.path)
+        else:
+            assert np.array_equal(alone.dist_all, r.dist_all)
+
+
+def test_slot_reuse_and_idle_restart(orc, graphs):
+    eng = {"grid": SlotEngine(graphs["grid"], 2, P.GpuOptions(drive=P.Drive.Persistent,
+                                                              heuristic=P.Heuristic.Scaled), astar=True)}
+    try:
+        srv = Server(eng, epoch_steps=0)            # epochs run to idle
+        for wave in range(3):                        # the engine goes idle between waves
+            reqs = [Request(i, 0.0, "grid", (37 * i + wave) % 3600, (1001 * i + 7 * wave) % 3600) for i in range(5)]
+            srv.run(reqs, time_limit_s=30)
+            for r in reqs:
+                g = graphs["grid"]
+                d, path, _, _ = orc.astar(g.row_offsets, g.col_indices, g.weights, g.x, g.y, g.h_scale,
+                                          orc.H_SCALED, r.src, r.dst, True)
+                assert r.dist == d and np.array_equal(r.path, path)
+    finally:
+        eng["grid"].close()
+
+
+def test_serving_blocks_other_runs(graphs):
+    e = SlotEngine(graphs["grid"], 2, P.GpuOptions(drive=P.Drive.Persistent), astar=True)
+    try:
+        with pytest.raises(RuntimeError):
+            e.sv.sssp(0, P.GpuOptions())
+    finally:
+        e.close()
