@@ -706,6 +706,7 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         if (h.steps >= h.maxSteps && next != PH_DONE) { atomicOr(&c->error, (u32)E_NOT_CONVERGED); next = PH_DONE; }
         if (h.epochEnd && h.steps >= h.epochEnd && next != PH_DONE) h.pause = 1u;
         h.phase = next;
+        s_next = next;
         // write back: header and counters (words 0-11), statistics, scan total, pause
         uint4* dst = reinterpret_cast<uint4*>(c);
         const uint4* src = reinterpret_cast<const uint4*>(&h);
@@ -717,6 +718,11 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         c->buckets = h.buckets;
         c->scanTotal = h.scanTotal;
         c->pause = h.pause;
+    }
+    __syncthreads();
+    if (s_next == PH_DONE) {   // no work left in any slot: every slot is finished
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x)
+            if (!r.slotDone[k]) r.slotDone[k] = 1u;
     }
 }
 
