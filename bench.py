@@ -54,12 +54,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="grid1000", choices=["grid1000", "rmat20"])
+    ap.add_argument("--workload", default="grid1000", choices=["grid1000", "rmat20", "union512"])
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-parameter autotuner")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="run the serving workload instead")
+    ap.add_argument("--serve-rate", type=float, default=1500.0)
+    ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--grid-slots", type=int, default=512)
+    ap.add_argument("--rmat-slots", type=int, default=24)
+    ap.add_argument("--epoch", type=int, default=48)
     args = ap.parse_args()
+    if args.serve:
+        return serve(args)
+    if args.workload == "union512":
+        return bench_union(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -113,6 +124,19 @@ def main():
     host_ref = P.GpuOptions(drive=P.Drive.Host, sync_every=8)          # reference defaults
     learned1 = opt(pick1["delta"], pick1["mapping"])
     learnedK = opt(pickK["delta"], pickK["mapping"])
+    tuned = None
+    if not args.no_tune:
+        from pathfinding_gpu_amd.autotune import autotune
+        log("[bench] autotuning launch parameters of the batched run")
+
+        def tfo(o):
+            return sv.multi_source(srcs, o, copy_back=False).ms
+
+        tuned = autotune(g, K, pickK["delta"], info["cu_count"], tfo, passes=1, reps=1, cache=None)
+        cand = tuned.options(info["cu_count"], delta=pickK["delta"])
+        if tuned.ms < tf_batch(pickK["delta"], pickK["mapping"]):
+            learnedK = cand
+        log(f"[bench] autotuned: {tuned}")
 
     # ---- warm-up, then timed steps: the learned batched run ----
     for _ in range(args.warmup):
@@ -161,7 +185,13 @@ def main():
                    "n": g.n, "m": g.m, "batch": K, "auto_delta": auto,
                    "learned_single": {"delta": pick1["delta"], "mapping": pick1["mapping"], "source": pick1["source"]},
                    "learned_batch": {"delta": pickK["delta"], "mapping": pickK["mapping"], "source": pickK["source"]},
-                   "drive": "persistent"},
+                   "drive": "persistent",
+                   "autotuned": None if tuned is None else {"block_size": learnedK.block_size,
+                                                            "wave_cutoff": learnedK.wave_cutoff,
+                                                            "block_cutoff": learnedK.block_cutoff,
+                                                            "grid_blocks": learnedK.grid_blocks,
+                                                            "mapping": int(learnedK.mapping),
+                                                            "trials": tuned.trials}},
         "plain_gteps": round(gteps_plain, 5),
         "plain_ms": round(ms_plain, 3),
         "speedup_vs_plain": round(gteps / gteps_plain, 3),
@@ -247,6 +277,115 @@ def main():
                                       "sample": f"cpuMultiSource over the first {kk} batch sources"}
         out["host"] = {"cpu_threads": os.cpu_count()}
     sv.close()
+    print(json.dumps(out))
+
+
+def bench_union(args):
+    """512 grid road graphs with sides uniform in [64, 512] (seeds 1000+i), one random source each, run as
+    one query over their disjoint union (one seed per graph) in a single launch: the persistent kernel, and
+    the captured hipGraph of 64 steps replayed until done."""
+    import pathfinding_gpu_amd as P
+    from pathfinding_gpu_amd import learned as L
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(4)
+    sides = rng.integers(64, 513, 512)
+    t0 = time.time()
+    gs = [P.grid_road(int(a), int(a), 1000 + i) for i, a in enumerate(sides)]
+    u, voff = P.union(gs)
+    local = np.array([int(rng.integers(0, gs[i].n)) for i in range(len(gs))], np.uint32)
+    seeds = (voff[:-1] + local).astype(np.uint32)
+    log(f"[union] {len(gs)} graphs, n={u.n} m={u.m} built in {time.time() - t0:.1f}s")
+    sv = P.GpuSolver(u)
+    pick = L.DeltaModel.load().choose(gs[int(np.argmax(sides))], 1)
+    auto = sv.auto_delta()
+    res = {}
+    for name, o in (("persistent_learned", P.GpuOptions(drive=P.Drive.Persistent, delta=pick["delta"],
+                                                         mapping=P.Mapping(pick["mapping"]))),
+                    ("graph_learned", P.GpuOptions(drive=P.Drive.Graph, sync_every=64, delta=pick["delta"],
+                                                   mapping=P.Mapping(pick["mapping"]))),
+                    ("persistent_davidson", P.GpuOptions(drive=P.Drive.Persistent, delta=auto))):
+        sv.multi_seed(seeds, o, copy_back=False)
+        ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
+        res[name] = (float(np.mean(ts)), r)
+    reached, arcs = sv.last_reach()
+    best = min(res, key=lambda k: res[k][0])
+    ms = res[best][0]
+    out = {"metric": "GTEPS, 512 uneven grid road graphs in one launch (disjoint union, one seed per graph)",
+           "value": round(arcs / (ms * 1e-3) / 1e9, 4), "unit": "GTEPS", "n_gpus": 1, "ms_per_step": round(ms, 3),
+           "steps": max(2, args.steps // 2), "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+           "config": {"workload": "512 grids, side U[64,512], seeds 1000+i", "n": u.n, "m": u.m, "best": best,
+                      "delta": pick["delta"], "mapping": pick["mapping"], "auto_delta": auto},
+           "variants_ms": {k: round(v[0], 3) for k, v in res.items()},
+           "graph_replays": res["graph_learned"][1].stats.launches}
+    if not args.no_cpu:
+        import oracle
+        r = sv.multi_seed(seeds, P.GpuOptions(drive=P.Drive.Graph, sync_every=64, delta=pick["delta"],
+                                              mapping=P.Mapping(pick["mapping"])))
+
+        def check(i):
+            g = gs[i]
+            d, _, pc, _, _ = oracle.dijkstra(g.row_offsets, g.col_indices, g.weights, int(local[i]))
+            sl = slice(int(voff[i]), int(voff[i + 1]))
+            par = r.parent[sl].astype(np.int64)
+            par[par != 0xFFFFFFFF] -= int(voff[i])
+            return bool(np.array_equal(r.dist[sl], d) and np.array_equal(par, pc.astype(np.int64)))
+
+        a = time.perf_counter()
+        with ThreadPoolExecutor(16) as ex:
+            ok = list(ex.map(check, range(len(gs))))
+        out["bitexact_instances"] = f"{sum(ok)}/{len(ok)}"
+        out["cpu_oracle_check_s"] = round(time.perf_counter() - a, 2)
+    sv.close()
+    print(json.dumps(out))
+
+
+def serve(args):
+    """Serving workload: a 2000x2000 grid road graph (seed 3) and an R-MAT scale-22 edge-factor-16 graph
+    (weights U[1,255], seed 22) resident in HBM; Poisson arrivals, 90% A* on the grid, 10% full SSSP on
+    the R-MAT, continuous batching into the persistent multi-slot kernel."""
+    import pathfinding_gpu_amd as P
+    from pathfinding_gpu_amd import learned as L
+    from pathfinding_gpu_amd.serving import Server, SlotEngine, poisson_requests
+
+    t0 = time.time()
+    grid = P.grid_road(2000, 2000, 3)
+    rmat = P.rmat(22, 16, 22)
+    log(f"[serve] graphs built in {time.time() - t0:.1f}s: grid n={grid.n} m={grid.m}, rmat n={rmat.n} m={rmat.m}")
+    model = L.DeltaModel.load()
+    pg = model.choose(grid, args.grid_slots)
+    pr = model.choose(rmat, args.rmat_slots)
+    eng = {
+        "grid": SlotEngine(grid, args.grid_slots, P.GpuOptions(drive=P.Drive.Persistent, delta=pg["delta"],
+                                                               mapping=P.Mapping(pg["mapping"]),
+                                                               heuristic=P.Heuristic.FloorEuclid), astar=True),
+        "rmat": SlotEngine(rmat, args.rmat_slots, P.GpuOptions(drive=P.Drive.Persistent, delta=pr["delta"],
+                                                               mapping=P.Mapping(pr["mapping"])), astar=False,
+                           keep_sssp=False),
+    }
+    reqs = poisson_requests(args.serve_rate, args.serve_requests, {"grid": 0.9, "rmat": 0.1},
+                            {"grid": grid.n, "rmat": rmat.n}, {"grid": True, "rmat": False}, seed=5)
+    warm = poisson_requests(args.serve_rate, 64, {"grid": 0.9, "rmat": 0.1}, {"grid": grid.n, "rmat": rmat.n},
+                            {"grid": True, "rmat": False}, seed=6)
+    srv = Server(eng, epoch_steps=args.epoch)
+    srv.run(warm, time_limit_s=120, log=log)
+    rep = srv.run(reqs, time_limit_s=600, log=log)
+    astar_done = [r for r in reqs if r.graph == "grid"]
+    out = {
+        "metric": "serving: A* + SSSP queries/s and p50/p99 latency (continuous batching), bit-exact",
+        "value": round(rep.qps, 1), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+        "offered_rate_qps": args.serve_rate, "requests": rep.requests, "wall_s": round(rep.wall_s, 3),
+        "latency_ms": {k: round(v, 2) for k, v in rep.latency_ms.items()},
+        "per_graph": rep.per_graph,
+        "astar_qps": round(len(astar_done) / rep.wall_s, 1),
+        "config": {"workload": "grid 2000x2000 seed 3 (90% A*, h=floor(Euclid)) + rmat s22 ef16 seed 22 (10% SSSP), "
+                               "Poisson arrivals", "grid_slots": args.grid_slots, "rmat_slots": args.rmat_slots,
+                   "epoch_steps": args.epoch, "grid_delta": pg["delta"], "grid_mapping": pg["mapping"],
+                   "rmat_delta": pr["delta"], "rmat_mapping": pr["mapping"]},
+        "dtype": "u32", "data": "synthetic",
+    }
+    for e in eng.values():
+        e.close()
     print(json.dumps(out))
 
 

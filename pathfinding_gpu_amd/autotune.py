@@ -1,0 +1,106 @@
+"""Autotuner for the engine's launch parameters (no reference counterpart).
+
+The reference fixes the block size (GpuOptions::blockSize = 256), the warp-tier
+cutoff (32, src/gpu_kernels.cuh:267), the CTA-tier cutoff (blockDim,
+src/gpu_kernels.cuh:250) and the persistent grid (src/gpu_sssp.cu:222-225).
+Here they are searched per graph and batch size on the device itself:
+
+  * block_size        -- workgroup size (64-lane waves per workgroup),
+  * wave_cutoff       -- degree from which one wave expands a vertex,
+  * block_cutoff      -- degree from which a whole workgroup expands a vertex,
+  * grid_blocks       -- workgroups of the persistent launch, i.e. how the
+                         frontier queue is chunked over the machine,
+  * mapping           -- thread / wave / hybrid.
+
+Search: coordinate descent from the engine defaults, each trial the best of
+`reps` timed runs of the caller's representative workload; results are cached
+per graph signature and batch size (JSON).  Delta is not tuned here: it is the
+learned network's job (learned.py).  Every setting gives bit-identical results.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import asdict, dataclass
+from typing import Callable, Dict, Optional
+
+from .graph import CSRGraph
+from .solver import Drive, GpuOptions, Mapping
+
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pathfinding_gpu_amd", "autotune.json")
+
+SPACE = {
+    "block_size": (128, 256, 512, 1024),
+    "wave_cutoff": (16, 32, 64, 128),
+    "block_cutoff_mult": (1, 4, 16),
+    "grid_mult": (0.5, 1, 2, 4),
+    "mapping": (0, 1, 2),
+}
+
+
+@dataclass
+class Tuned:
+    block_size: int = 256
+    wave_cutoff: int = 64
+    block_cutoff_mult: int = 1
+    grid_mult: float = 2
+    mapping: int = 2
+    ms: float = float("inf")
+    trials: int = 0
+
+    def options(self, cus: int, **kw) -> GpuOptions:
+        return GpuOptions(drive=Drive.Persistent, block_size=self.block_size, wave_cutoff=self.wave_cutoff,
+                          block_cutoff=self.block_size * self.block_cutoff_mult,
+                          grid_blocks=max(1, int(self.grid_mult * cus)), mapping=Mapping(self.mapping), **kw)
+
+
+def signature(g: CSRGraph, batch: int, delta: int) -> str:
+    return f"{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
+
+
+def autotune(g: CSRGraph, batch: int, delta: int, cus: int, time_fn: Callable[[GpuOptions], float],
+             passes: int = 2, reps: int = 2, cache: Optional[str] = CACHE, log=None) -> Tuned:
+    """time_fn(options) -> milliseconds of one representative run (options carry delta)."""
+    key = signature(g, batch, delta)
+    db: Dict[str, dict] = {}
+    if cache and os.path.exists(cache):
+        try:
+            db = json.load(open(cache))
+        except (OSError, ValueError):
+            db = {}
+        if key in db:
+            return Tuned(**db[key])
+    best = Tuned()
+
+    def measure(t: Tuned) -> float:
+        o = t.options(cus, delta=delta)
+        return min(time_fn(o) for _ in range(reps))
+
+    best.ms = measure(best)
+    trials = 1
+    for _ in range(passes):
+        improved = False
+        for name, values in SPACE.items():
+            for v in values:
+                if getattr(best, name) == v:
+                    continue
+                cand = Tuned(**{**asdict(best), name: v})
+                ms = measure(cand)
+                trials += 1
+                if log:
+                    log(f"[autotune] {name}={v}: {ms:.3f} ms (best {best.ms:.3f})")
+                if ms < best.ms * 0.98:
+                    cand.ms = ms
+                    best = cand
+                    improved = True
+        if not improved:
+            break
+    best.trials = trials
+    if cache:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        db[key] = asdict(best)
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1)
+        os.replace(tmp, cache)
+    return best

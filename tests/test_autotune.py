@@ -1,0 +1,36 @@
+"""Autotuner search logic on the CPU with a synthetic cost function (no GPU)."""
+import json
+
+import pathfinding_gpu_amd as P
+from pathfinding_gpu_amd.autotune import SPACE, Tuned, autotune
+
+
+def test_coordinate_descent_finds_the_optimum(tmp_path):
+    g = P.grid_road(10, 10, 1)
+    calls = []
+
+    def cost(o):
+        calls.append(o)
+        # separable bowl with its minimum at bs=512, wave cutoff 16, block cutoff 4x, grid 1x CUs, thread mapping
+        return (1.0 + abs(o.block_size - 512) / 512 + abs(o.wave_cutoff - 16) / 64
+                + abs(o.block_cutoff / o.block_size - 4) / 4 + abs(o.grid_blocks - 256) / 256 + int(o.mapping) * 0.3)
+
+    cache = str(tmp_path / "tune.json")
+    t = autotune(g, 1, 123, 256, cost, passes=3, reps=1, cache=cache)
+    assert (t.block_size, t.wave_cutoff, t.block_cutoff_mult, t.grid_mult, t.mapping) == (512, 16, 4, 1, 0)
+    assert t.trials == len(calls)
+    # every trial carried the caller's Delta and a persistent drive
+    assert all(o.delta == 123 and int(o.drive) == 2 for o in calls)
+    # cached: a second call does not measure again
+    n = len(calls)
+    t2 = autotune(g, 1, 123, 256, cost, cache=cache)
+    assert len(calls) == n and t2.block_size == 512
+    db = json.load(open(cache))
+    assert len(db) == 1
+
+
+def test_options_mapping():
+    o = Tuned(block_size=1024, wave_cutoff=32, block_cutoff_mult=4, grid_mult=0.5, mapping=1).options(256, delta=7)
+    assert (o.block_size, o.wave_cutoff, o.block_cutoff, o.grid_blocks, int(o.mapping), o.delta) == \
+        (1024, 32, 4096, 128, 1, 7)
+    assert all(len(v) >= 3 for v in SPACE.values())
