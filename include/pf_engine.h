@@ -181,6 +181,9 @@ int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end); /* n 
 int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h); /* n entries */
 int pf_debug_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* total);
 int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w); /* the uploaded CSR, m each */
+/* Streaming copy of `bytes` (read once, written once) `reps` times; *ms per copy.
+ * Calibrates HBM byte counters and gives the achievable copy bandwidth. */
+int pf_debug_copy(uint64_t bytes, uint32_t reps, double* ms);
 
 /* ---- learned Delta: a small MLP with HIP inference / training kernels --------
  * No reference counterpart (the reference has only the Davidson default,

@@ -79,6 +79,7 @@ ENGINE_SYMBOLS = {
     "pf_debug_heuristic": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, u32p]),
     "pf_debug_exclusive_scan": (C.c_int, [u32p, u32p, C.c_uint32, u32p]),
     "pf_debug_sorted_rows": (C.c_int, [C.c_void_p, u32p, u32p]),
+    "pf_debug_copy": (C.c_int, [C.c_uint64, C.c_uint32, dblp]),
     "pf_serve_begin": (C.c_int, [C.c_void_p, C.POINTER(PfOptions), C.c_int]),
     "pf_serve_admit": (C.c_int, [C.c_void_p, u32p, u32p, u32p, C.c_uint32]),
     "pf_serve_run": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, C.POINTER(PfStats)]),

@@ -1121,6 +1121,31 @@ int pf_debug_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t count, u
     return PF_OK;
 }
 
+int pf_debug_copy(uint64_t bytes, uint32_t reps, double* ms) {
+    const u64 words = std::max<u64>(1, bytes / 16);
+    uint4 *a = nullptr, *b = nullptr;
+    PF_HIP(hipMalloc((void**)&a, words * 16));
+    PF_HIP(hipMalloc((void**)&b, words * 16));
+    PF_HIP(hipMemset(a, 1, words * 16));
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    const u32 grid = std::min<u32>(divUp(words, 256), 1u << 16);
+    hipLaunchKernelGGL(copyKernel, dim3(grid), dim3(256), 0, 0, a, b, words);
+    PF_HIP(hipEventRecord(e0, 0));
+    for (u32 i = 0; i < reps; ++i) hipLaunchKernelGGL(copyKernel, dim3(grid), dim3(256), 0, 0, a, b, words);
+    PF_HIP(hipEventRecord(e1, 0));
+    PF_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    PF_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / std::max(1u, reps);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return PF_OK;
+}
+
 int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w) {
     int rc = setDev(s);
     if (rc) return rc;

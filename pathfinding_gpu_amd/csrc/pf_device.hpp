@@ -1093,6 +1093,12 @@ __global__ void reachKernel(const u64* state, const u32* row, u32 n, u64 total, 
     }
 }
 
+// Plain streaming copy (counter calibration: reads and writes exactly `words` 16-byte words).
+__global__ void copyKernel(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 words) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (u64)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Standalone exclusive scan (test hook for the compaction arithmetic):
 // one block, serial chunks per thread.
 __global__ void exclusiveScanKernel(const u32* in, u32* out, u32 count, u32* total) {
