@@ -134,6 +134,26 @@ __device__ __forceinline__ void stAgent64(unsigned long long* p, unsigned long l
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Every word that one workgroup writes and another reads during a run goes
+// through these agent-scope relaxed atomics (global_load / global_store with
+// device scope): they are coherent across the XCDs' L2 caches without any
+// cache write-back or invalidation, so read-only graph data and
+// stale-tolerant reads keep their L2 lines from step to step.
+template <class T>
+__device__ __forceinline__ T ldA(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+struct NoDeduce {
+    using type = T;
+};
+template <class T>
+__device__ __forceinline__ void stA(T* p, typename NoDeduce<T>::type v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// All of this wave's memory operations have completed (stores visible device-wide).
+__device__ __forceinline__ void waitAll() { __builtin_amdgcn_s_waitcnt(0); }
+
 __device__ __forceinline__ u32 laneId() { return __lane_id(); }
 
 // slot k of element k*n+v
@@ -155,7 +175,7 @@ __device__ __forceinline__ u32 waveAggInc(u32* ctr) {
     base = __builtin_amdgcn_readfirstlane(base);
     return base + rank;
 }
-__device__ __forceinline__ void pushQ(u32* data, u32* ctr, u32 item) { data[waveAggInc(ctr)] = item; }
+__device__ __forceinline__ void pushQ(u32* data, u32* ctr, u32 item) { stA(&data[waveAggInc(ctr)], item); }
 
 // Read-then-atomicOr dedup bit (src/gpu_kernels.cuh:124-129).
 __device__ __forceinline__ bool testAndSetBit(u32* mask, u32 i) {
@@ -269,7 +289,7 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
     if (relaxAtomic<AM>(&r.state[gv], (u32)nd, it.u, ls)) {
         ls.updates++;
         if (o.flags) {
-            o.flags[gv] = 1u;
+            stA(&o.flags[gv], 1u);
             ls.pushes++;
         } else if (key < it.bnd) {
             if (testAndSetBit(o.nearMask, gv)) { pushQ(o.nearQ, o.nearCnt, gv); ls.pushes++; }
@@ -324,11 +344,11 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
         it.u = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
         it.ub = NOKEY; it.bnd = NOKEY;
         if (idx < size) {
-            const u32 gi = in[idx];
+            const u32 gi = ldA(&in[idx]);
             if (inMask) clearBit(inMask, gi);          // may re-enter the frontier later
             it.k = slotOf(r, gi);
             it.u = gi - it.k * n;
-            it.du = distOf(r.state[gi]);
+            it.du = distOf(ldA(&r.state[gi]));         // the latest distance: expanded with its final value
             bool active = it.du != INF;
             if constexpr (ASTAR) {
                 const u32 t = r.target[it.k];
@@ -342,7 +362,7 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                 if (sel == 0) { it.beg = r0; it.end = r1; }
                 else if (sel == 1) { it.beg = r0; it.end = r.g.lightEnd[it.u]; }
                 else { it.beg = r.g.lightEnd[it.u]; it.end = r1; }
-                it.bnd = useBound ? r.bound[it.k] : NOKEY;
+                it.bnd = useBound ? ldA(&r.bound[it.k]) : NOKEY;
             }
         }
         bool done = it.end <= it.beg;
@@ -432,10 +452,10 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
     (void)ubState;
     if constexpr (ASTAR) t = loadTarget(r, r.target[0]);
     for (u32 u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
-        if (!actIn[u]) continue;
-        actIn[u] = 0;
+        if (!ldA(&actIn[u])) continue;
+        stA(&actIn[u], (u8)0);
         Item it;
-        it.u = u; it.k = 0; it.du = distOf(r.state[u]);
+        it.u = u; it.k = 0; it.du = distOf(ldA(&r.state[u]));
         it.bnd = NOKEY; it.ub = NOKEY; it.t = t;
         bool active = it.du != INF;
         if constexpr (ASTAR) {
@@ -454,7 +474,7 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
             }
             if (relaxAtomic<AM>(&r.state[a.x], (u32)nd, u, ls)) {
                 ls.updates++;
-                actOut[a.x] = 1;
+                stA(&actOut[a.x], (u8)1);
                 if (ldAgent(changed) == 0u) stAgent(changed, 1u);
             }
         }
@@ -469,9 +489,9 @@ __device__ void farMin(const Run& r, const u32* far, u32 size) {
         unsigned long long key = NOKEY;
         u32 k = 0;
         if (idx < size) {
-            const u32 gi = far[idx];
+            const u32 gi = ldA(&far[idx]);
             k = slotOf(r, gi);
-            const u32 d = distOf(r.state[gi]);
+            const u32 d = distOf(ldA(&r.state[gi]));
             if (d != INF) {
                 u64 kk = d;
                 bool ok = true;
@@ -480,7 +500,7 @@ __device__ void farMin(const Run& r, const u32* far, u32 size) {
                     kk += heur(r, loadTarget(r, t), gi - k * n);
                     ok = kk <= (u64)distOf(r.state[k * n + t]);
                 }
-                if (ok && kk >= r.bound[k]) key = kk;   // below the old bound: already settled in that bucket
+                if (ok && kk >= ldA(&r.bound[k])) key = kk;   // below the old bound: already settled in that bucket
             }
         }
         if (r.K == 1) {                              // one slot: wave minimum, one atomic per wave
@@ -502,9 +522,9 @@ __device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ,
                              u32* farOut, u32* farOutCnt) {
     const u32 n = r.g.n;
     for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size; idx += gridDim.x * blockDim.x) {
-        const u32 gi = far[idx];
+        const u32 gi = ldA(&far[idx]);
         const u32 k = slotOf(r, gi);
-        const u32 d = distOf(r.state[gi]);
+        const u32 d = distOf(ldA(&r.state[gi]));
         u64 key = d;
         bool ok = d != INF;
         if constexpr (ASTAR) {
@@ -512,10 +532,10 @@ __device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ,
             key += heur(r, loadTarget(r, t), gi - k * n);
             ok = ok && key <= (u64)distOf(r.state[k * n + t]);
         }
-        ok = ok && key >= r.bound[k];
+        ok = ok && key >= ldA(&r.bound[k]);
         if (!ok) {
             clearBit(r.farMask, gi);                 // stale or pruned
-        } else if (key < r.newBound[k]) {
+        } else if (key < ldA(&r.newBound[k])) {
             clearBit(r.farMask, gi);                 // leaves the far pile for the new bucket
             if (testAndSetBit(nearMask, gi)) pushQ(nearQ, nearCnt, gi);
         } else {
@@ -554,11 +574,11 @@ __device__ void scanTiles(const Run& r, u32* waveTotals) {
         u32 s = 0;
         for (u32 j = 0; j < per; ++j) {
             const u32 i = b + j * blockDim.x + threadIdx.x;
-            if (i < r.total) s += r.flags[i];
+            if (i < r.total) s += ldA(&r.flags[i]);
         }
         u32 ex;
         const u32 tot = blockExclusiveScan(s, waveTotals, ex);
-        if (threadIdx.x == 0) r.tileSums[t] = tot;
+        if (threadIdx.x == 0) stA(&r.tileSums[t], tot);
     }
 }
 
@@ -566,16 +586,16 @@ __device__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
     const u32 per = r.tileItems / blockDim.x;
     for (u32 t = blockIdx.x; t < r.numTiles; t += gridDim.x) {
         const u32 b = t * r.tileItems;
-        u32 base = r.tileSums[t];
+        u32 base = ldA(&r.tileSums[t]);
         // items are visited in ascending order: chunk j covers [b + j*bs, b + (j+1)*bs)
         for (u32 j = 0; j < per; ++j) {
             const u32 i = b + j * blockDim.x + threadIdx.x;
-            const u32 f = i < r.total ? r.flags[i] : 0u;
+            const u32 f = i < r.total ? ldA(&r.flags[i]) : 0u;
             u32 ex;
             const u32 tot = blockExclusiveScan(f, waveTotals, ex);
             if (f) {
-                outQ[base + ex] = i;
-                r.flags[i] = 0u;
+                stA(&outQ[base + ex], i);
+                stA(&r.flags[i], 0u);
             }
             base += tot;
         }
@@ -593,7 +613,7 @@ __device__ void controller(const Run& r, u32* waveTotals) {
     Ctl* c = r.ctl;
     __shared__ __align__(16) Ctl h;           // working copy
     __shared__ u32 s_next, s_resetMin, s_setBounds, s_scan, s_part;
-    if (threadIdx.x < sizeof(Ctl) / 16) reinterpret_cast<uint4*>(&h)[threadIdx.x] = reinterpret_cast<const uint4*>(c)[threadIdx.x];
+    if (threadIdx.x < sizeof(Ctl) / 4) reinterpret_cast<u32*>(&h)[threadIdx.x] = ldA(reinterpret_cast<const u32*>(c) + threadIdx.x);
     __syncthreads();
     if (threadIdx.x == 0) {
         const u32 phase = h.phase;
@@ -663,21 +683,21 @@ __device__ void controller(const Run& r, u32* waveTotals) {
     }
     __syncthreads();
     if (s_resetMin) {
-        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.minKey[k] = NOKEY;
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) stA(&r.minKey[k], (unsigned long long)NOKEY);
     }
     if (s_part) {   // the partition has used old and new bounds; the new ones take over
-        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) r.bound[k] = r.newBound[k];
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) stA(&r.bound[k], ldA(&r.newBound[k]));
     }
     if (s_setBounds) {
         bool any = false;
         for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) {
-            const unsigned long long mk = r.minKey[k];
+            const unsigned long long mk = ldA(&r.minKey[k]);
             if (mk != NOKEY) {
-                r.newBound[k] = (mk / r.delta + 1) * (u64)r.delta;   // src/gpu_sssp.cu:441
+                stA(&r.newBound[k], (mk / r.delta + 1) * (u64)r.delta);   // src/gpu_sssp.cu:441
                 any = true;
             } else {
-                r.newBound[k] = r.bound[k];
-                if (!r.slotDone[k]) r.slotDone[k] = 1u;
+                stA(&r.newBound[k], ldA(&r.bound[k]));
+                if (!ldA(&r.slotDone[k])) stA(&r.slotDone[k], 1u);
             }
         }
         const int anyAll = __syncthreads_or(any);
@@ -692,11 +712,11 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         const u32 b = threadIdx.x * per;
         u32 sum = 0;
         for (u32 j = 0; j < per; ++j)
-            if (b + j < nt) sum += r.tileSums[b + j];
+            if (b + j < nt) sum += ldA(&r.tileSums[b + j]);
         u32 ex;
         const u32 tot = blockExclusiveScan(sum, waveTotals, ex);
         for (u32 j = 0; j < per; ++j)
-            if (b + j < nt) { const u32 v = r.tileSums[b + j]; r.tileSums[b + j] = ex; ex += v; }
+            if (b + j < nt) { const u32 v = ldA(&r.tileSums[b + j]); stA(&r.tileSums[b + j], ex); ex += v; }
         if (threadIdx.x == 0) h.scanTotal = tot;
     }
     __syncthreads();
@@ -707,22 +727,21 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         if (h.epochEnd && h.steps >= h.epochEnd && next != PH_DONE) h.pause = 1u;
         h.phase = next;
         s_next = next;
-        // write back: header and counters (words 0-11), statistics, scan total, pause
-        uint4* dst = reinterpret_cast<uint4*>(c);
-        const uint4* src = reinterpret_cast<const uint4*>(&h);
-        dst[0] = src[0];
-        dst[1] = src[1];
-        dst[2] = src[2];
-        c->steps = h.steps;
-        c->iters = h.iters;
-        c->buckets = h.buckets;
-        c->scanTotal = h.scanTotal;
-        c->pause = h.pause;
+    }
+    __syncthreads();
+    // write back: header and counters (words 0-11), statistics, scan total, pause
+    if (threadIdx.x < 12) stA(reinterpret_cast<u32*>(c) + threadIdx.x, reinterpret_cast<const u32*>(&h)[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        stA(&c->steps, h.steps);
+        stA(&c->iters, h.iters);
+        stA(&c->buckets, h.buckets);
+        stA(&c->scanTotal, h.scanTotal);
+        stA(&c->pause, h.pause);
     }
     __syncthreads();
     if (s_next == PH_DONE) {   // no work left in any slot: every slot is finished
         for (u32 k = threadIdx.x; k < r.K; k += blockDim.x)
-            if (!r.slotDone[k]) r.slotDone[k] = 1u;
+            if (!ldA(&r.slotDone[k])) stA(&r.slotDone[k], 1u);
     }
 }
 
@@ -814,13 +833,13 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             default: break;
         }
         // ---- arrive; the last workgroup runs the controller ----
-        // Each wave's stores complete at the workgroup barrier; thread 0 then publishes them
-        // device-wide (agent release) and takes a ticket.  In persistent mode the ticket is a
-        // monotonic 64-bit counter: barrier number and "last" follow from the ticket alone, and
+        // Each wave waits until its own memory operations have completed (its atomic stores are
+        // then visible device-wide); thread 0 then takes a ticket.  In persistent mode the ticket is
+        // a monotonic 64-bit counter: barrier number and "last" follow from the ticket alone, and
         // nothing resets it while other workgroups may still be adding to it.
+        waitAll();
         __syncthreads();
         if (threadIdx.x == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (persistent) {
                 const unsigned long long t = atomicAdd(&r.bar[0], 1ull);
                 s_bar = t / gridDim.x;
@@ -829,31 +848,29 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
                 const u32 t = atomicAdd(&c->arrive, 1u);
                 s_last = (t == gridDim.x - 1) ? 1u : 0u;
             }
-            if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (s_last) {
             controller(r, waveTotals);
+            waitAll();
             __syncthreads();
             if (threadIdx.x == 0) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (persistent) stAgent64(&r.bar[BAR_GEN], s_bar + 1ull);
-                else stAgent(&c->arrive, 0u);   // next launch starts after this kernel ends
+                if (persistent) stA(&r.bar[BAR_GEN], s_bar + 1ull);
+                else stA(&c->arrive, 0u);   // next launch starts after this kernel ends
             }
         } else if (persistent) {
             if (threadIdx.x == 0) {
-                // relaxed agent-scope polling, bounded by wall-clock time; one acquire after the wait
+                // relaxed agent-scope polling, bounded by wall-clock time
                 const unsigned long long t0 = wall_clock64();
-                while (ldAgent64(&r.bar[BAR_GEN]) <= s_bar) {
+                while (ldA(&r.bar[BAR_GEN]) <= s_bar) {
                     __builtin_amdgcn_s_sleep(8);
                     if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); break; }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();
         }
         if (!persistent) break;
-        if (threadIdx.x == 0) s_exit = ldAgent(&c->pause) | (ldAgent(&c->error) & E_BARRIER);
+        if (threadIdx.x == 0) s_exit = ldA(&c->pause) | (ldA(&c->error) & E_BARRIER);
         __syncthreads();
         if (s_exit) break;
     }
