@@ -81,7 +81,7 @@ struct pf_solver {
 
     u32* d_row = nullptr;
     uint2* d_adj = nullptr;
-    u32* d_lightEnd = nullptr;
+    uint2* d_vrec = nullptr;   // {row_offsets[u], light end of row u} [n+1]
     int2* d_xyi = nullptr;
     double2* d_xyd = nullptr;
 
@@ -182,7 +182,7 @@ void sanitize(pf_options& o) {
 int ensureLightEnd(pf_solver* s, u32 delta) {
     if (delta == s->lightDelta) return PF_OK;
     hipLaunchKernelGGL(lightEndKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, s->d_row, s->d_adj, s->n, delta,
-                       s->d_lightEnd);
+                       s->d_vrec);
     PF_HIP(hipGetLastError());
     PF_HIP(hipStreamSynchronize(s->s));
     s->lightDelta = delta;
@@ -286,7 +286,7 @@ int ensureFlags(pf_solver* s, u32 tileItems) {
 
 Run baseRun(const pf_solver* s) {
     Run run{};
-    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
     run.state = s->d_state;
     run.ctl = s->d_ctl;
     run.stats = s->d_stats;
@@ -355,7 +355,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         PF_HIP(hipMemcpyAsync(s->d_target, spec.targets, (size_t)spec.k * 4, hipMemcpyHostToDevice, s->s));
 
     Run run{};
-    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
     run.state = s->d_state;
     run.ctl = s->d_ctl;
     run.stats = s->d_stats;
@@ -698,7 +698,7 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
     auto t0 = std::chrono::steady_clock::now();
     // ---- CSR upload: interleaved {dst, w}, pinned staging, separate streams ----
     if ((rc = s->dalloc(&s->d_row, (u64)s->n + 1)) || (rc = s->dalloc(&s->d_adj, s->m)) ||
-        (rc = s->dalloc(&s->d_lightEnd, s->n)))
+        (rc = s->dalloc(&s->d_vrec, (u64)s->n + 1)))
         return cleanupFail(rc);
     {
         const size_t bRow = (size_t)(s->n + 1) * 4, bAdj = (size_t)s->m * 8;
@@ -748,6 +748,12 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
                 return cleanupFail(fail(PF_ERR_HIP, "coords upload"));
         }
     }
+    // vertex records valid from the start (every arc light); Delta runs recompute the split
+    hipLaunchKernelGGL(lightEndKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, s->d_row, s->d_adj, s->n,
+                       0xFFFFFFFFu, s->d_vrec);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->s) != hipSuccess)
+        return cleanupFail(fail(PF_ERR_HIP, "vertex records"));
+    s->lightDelta = 0xFFFFFFFFu;
     s->uploadMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
     // ---- working buffers (allocated once, src/gpu_sssp.cu:162-186) ----
@@ -782,7 +788,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
@@ -1080,7 +1086,9 @@ int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end) {
     s->lightDelta = 0;
     rc = ensureLightEnd(s, delta);
     if (rc) return rc;
-    PF_HIP(hipMemcpy(light_end, s->d_lightEnd, (size_t)s->n * 4, hipMemcpyDeviceToHost));
+    std::vector<uint2> vr(s->n);
+    PF_HIP(hipMemcpy(vr.data(), s->d_vrec, (size_t)s->n * 8, hipMemcpyDeviceToHost));
+    for (u32 v = 0; v < s->n; ++v) light_end[v] = vr[v].y;
     return PF_OK;
 }
 
@@ -1091,7 +1099,7 @@ int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h) {
     if (kind == PF_H_FLOOR_EUCLID && !s->intCoords) return fail(PF_ERR_INVALID, "no integer coordinates");
     if (kind == PF_H_SCALED && !s->hasCoords) return fail(PF_ERR_INVALID, "no coordinates");
     Run run{};
-    run.g = DevGraph{s->d_row, s->d_adj, s->d_lightEnd, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
     run.hScale = s->hScale;
     run.hkind = (u32)kind;
     u32* d_h = nullptr;

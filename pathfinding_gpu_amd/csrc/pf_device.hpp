@@ -69,7 +69,8 @@ static_assert(sizeof(Ctl) == 128, "control block is eight 16-byte words");
 struct DevGraph {
     const u32* row;       // row_offsets [n+1]
     const uint2* adj;     // {dst, w} [m], each row sorted by (w, dst)
-    const u32* lightEnd;  // [n] first heavy arc of each row (for the run's delta)
+    const uint2* vrec;    // [n+1] {row_offsets[u], first heavy arc of row u for the run's delta}:
+                          // one 8-byte record per vertex, so expanding u touches one line
     const int2* xyi;      // integer coordinates [n] (FLOOR_EUCLID) or null
     const double2* xyd;   // double coordinates [n] (SCALED) or null
     u32 n, m;
@@ -357,11 +358,13 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                 if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
             }
             if (active) {
-                if (markR && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &r.ctl->cnt[C_R], gi);
-                const u32 r0 = r.g.row[it.u], r1 = r.g.row[it.u + 1];
+                const uint2 va = r.g.vrec[it.u], vb = r.g.vrec[it.u + 1];
+                const u32 r0 = va.x, le = va.y, r1 = vb.x;
+                // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
+                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &r.ctl->cnt[C_R], gi);
                 if (sel == 0) { it.beg = r0; it.end = r1; }
-                else if (sel == 1) { it.beg = r0; it.end = r.g.lightEnd[it.u]; }
-                else { it.beg = r.g.lightEnd[it.u]; it.end = r1; }
+                else if (sel == 1) { it.beg = r0; it.end = le; }
+                else { it.beg = le; it.end = r1; }
                 it.bnd = useBound ? ldA(&r.bound[it.k]) : NOKEY;
             }
         }
@@ -463,8 +466,8 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
             if (active && (u64)it.du + heur(r, t, u) > it.ub) active = false;
         }
         if (!active) continue;
-        const u32 r1 = r.g.row[u + 1];
-        for (u32 e = r.g.row[u]; e < r1; ++e) {
+        const u32 r1 = r.g.vrec[u + 1].x;
+        for (u32 e = r.g.vrec[u].x; e < r1; ++e) {
             const uint2 a = r.g.adj[e];
             const u64 nd = (u64)it.du + a.y;
             ls.edges++;
@@ -1043,7 +1046,7 @@ __global__ void chaseSlotsKernel(const u64* state, u32 n, const u32* slots, cons
 }
 
 // lightEndKernel (src/gpu_kernels.cuh:361-371).
-__global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, u32* lightEnd) {
+__global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, uint2* vrec) {
     const u32 u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u < n) {
         u32 lo = row[u], hi = row[u + 1];
@@ -1051,8 +1054,9 @@ __global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delt
             const u32 mid = lo + ((hi - lo) >> 1);
             if (adj[mid].y <= delta) lo = mid + 1; else hi = mid;
         }
-        lightEnd[u] = lo;
+        vrec[u] = make_uint2(row[u], lo);
     }
+    if (u == 0) vrec[n] = make_uint2(row[n], row[n]);
 }
 
 // heuristicKernel (src/gpu_kernels.cuh:355-358) -- test hook; the engine
