@@ -75,6 +75,10 @@ typedef struct {
  * PF_GRAPH_SORT_ROWS: the engine then sorts each row by (w, dst) on the host
  * before upload. */
 #define PF_GRAPH_SORT_ROWS 1u
+/* The engine renumbers graphs that have coordinates internally (Morton order) for
+ * cache locality; every id in and out of the C-ABI stays the caller's.  This flag
+ * keeps the caller's numbering inside the engine too. */
+#define PF_GRAPH_NO_REORDER 2u
 typedef struct {
     uint32_t n, m;
     const uint32_t* row_offsets; /* n+1 */

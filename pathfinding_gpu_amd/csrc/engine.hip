@@ -81,7 +81,10 @@ struct pf_solver {
 
     u32* d_row = nullptr;
     uint2* d_adj = nullptr;
-    uint2* d_vrec = nullptr;   // {row_offsets[u], light end of row u} [n+1]
+    uint4* d_vrec = nullptr;   // {row_offsets[u], light end of row u, original id, 0} [n+1]
+    u32* d_inv = nullptr;      // original id -> internal id (null: identity)
+    std::vector<u32> perm, inv; // internal -> original, original -> internal (empty: identity)
+    std::vector<u32> callerRow; // the caller's row_offsets (test hooks report in the caller's CSR)
     int2* d_xyi = nullptr;
     double2* d_xyd = nullptr;
 
@@ -147,7 +150,7 @@ namespace {
 // src/gpu_sssp.cu:113-127, for this engine's buffers).
 u64 estimateBytes(u32 n, u32 m, u32 K, bool coords) {
     const u64 Kn = (u64)K * n;
-    return (u64)(n + 1) * 4 + (u64)m * 8 + (u64)n * 4 + (coords ? (u64)n * 24 : 0)   // CSR, lightEnd, coords
+    return (u64)(n + 1) * 20 + (u64)m * 8 + (coords ? (u64)n * 28 : 0)   // CSR + vertex records, coords + inv
            + Kn * 8                                                                    // state
            + 2 * Kn * 4 + 2 * Kn * 4 + Kn * 4                                          // near x2, far x2, R
            + 4 * maskWords(Kn) * 4 + (u64)n * 2 + (u64)K * 64;                          // masks, naive flags, slots
@@ -164,6 +167,9 @@ void sortRows(u32 n, const u32* row, u32* col, u32* w) {
         for (u32 k = b; k < e; ++k) { w[k] = (u32)(tmp[k - b] >> 32); col[k] = (u32)tmp[k - b]; }
     }
 }
+
+// caller's vertex id -> internal id
+inline u32 toIn(const pf_solver* s, u32 v) { return s->inv.empty() ? v : s->inv[v]; }
 
 int setDev(const pf_solver* s) {
     PF_HIP(hipSetDevice(s->dev));
@@ -350,9 +356,25 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     }
     rc = ensureSeeds(s, spec.nseeds);
     if (rc) return rc;
-    PF_HIP(hipMemcpyAsync(s->d_seeds, spec.seeds, (size_t)spec.nseeds * 4, hipMemcpyHostToDevice, s->s));
-    if (spec.astar)
-        PF_HIP(hipMemcpyAsync(s->d_target, spec.targets, (size_t)spec.k * 4, hipMemcpyHostToDevice, s->s));
+    const u32* seeds = spec.seeds;
+    const u32* targets = spec.targets;
+    std::vector<u32> seedsIn, targetsIn;   // internal numbering
+    if (!s->inv.empty()) {
+        seedsIn.resize(spec.nseeds);
+        for (u32 i = 0; i < spec.nseeds; ++i) {
+            const u32 k = spec.seeds[i] / s->n, v = spec.seeds[i] - k * s->n;
+            seedsIn[i] = k * s->n + s->inv[v];
+        }
+        seeds = seedsIn.data();
+        if (spec.astar) {
+            targetsIn.resize(spec.k);
+            for (u32 i = 0; i < spec.k; ++i) targetsIn[i] = s->inv[spec.targets[i]];
+            targets = targetsIn.data();
+        }
+    }
+    PF_HIP(hipMemcpyAsync(s->d_seeds, seeds, (size_t)spec.nseeds * 4, hipMemcpyHostToDevice, s->s));
+    if (spec.astar) PF_HIP(hipMemcpyAsync(s->d_target, targets, (size_t)spec.k * 4, hipMemcpyHostToDevice, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));   // the host vectors above go out of scope
 
     Run run{};
     run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
@@ -504,9 +526,14 @@ int copyBack(pf_solver* s, u32 k, u32* dist, u32* parent) {
     const u64 Kn = (u64)k * s->n;
     std::vector<u64> hst(Kn);
     PF_HIP(hipMemcpy(hst.data(), s->d_state, Kn * 8, hipMemcpyDeviceToHost));
-    for (u64 i = 0; i < Kn; ++i) {
-        if (dist) dist[i] = (u32)(hst[i] >> 32);
-        if (parent) parent[i] = (u32)hst[i];
+    const u32 n = s->n;
+    const u32* perm = s->perm.empty() ? nullptr : s->perm.data();
+#pragma omp parallel for schedule(static) if (Kn > (1u << 20))
+    for (long long i = 0; i < (long long)Kn; ++i) {
+        const u64 k = (u64)i / n, vi = (u64)i - k * n;
+        const u64 o = k * n + (perm ? perm[vi] : vi);   // parents are already caller ids
+        if (dist) dist[o] = (u32)(hst[i] >> 32);
+        if (parent) parent[o] = (u32)hst[i];
     }
     return PF_OK;
 }
@@ -522,8 +549,8 @@ int chasePaths(pf_solver* s, const u32* srcs, const u32* dsts, u32 k, u32* tdist
     }
     PF_HIP(hipMemcpyAsync(s->d_src, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s->s));
     PF_HIP(hipMemcpyAsync(s->d_target, dsts, (size_t)k * 4, hipMemcpyHostToDevice, s->s));
-    hipLaunchKernelGGL(chaseKernel, dim3(divUp(k, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_src, s->d_target, k,
-                       s->d_paths, cap, s->d_lens, s->d_tdist);
+    hipLaunchKernelGGL(chaseKernel, dim3(divUp(k, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_inv, s->d_src,
+                       s->d_target, k, s->d_paths, cap, s->d_lens, s->d_tdist);
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(tdist, s->d_tdist, (size_t)k * 4, hipMemcpyDeviceToHost, s->s));
     PF_HIP(hipMemcpyAsync(lens, s->d_lens, (size_t)k * 4, hipMemcpyDeviceToHost, s->s));
@@ -696,37 +723,119 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
     }
     int rc;
     auto t0 = std::chrono::steady_clock::now();
+    // ---- vertex order: Morton (Z-order) renumbering for graphs with coordinates ----
+    // Neighbouring vertices then share cache lines of the state, vertex records and
+    // arcs.  Internal only: every id crossing the C-ABI is an original id, and the
+    // packed state stores the original parent id, so the min-index parent rule and
+    // all results are those of the original numbering.
+    bool reorder = coords && !(g->flags & PF_GRAPH_NO_REORDER) && s->n > 1;
+    double x0 = 0.0, x1 = 0.0, y0 = 0.0, y1 = 0.0;
+    if (reorder) {
+        x0 = x1 = g->x[0];
+        y0 = y1 = g->y[0];
+        for (u32 v = 0; v < s->n && reorder; ++v) {
+            if (!std::isfinite(g->x[v]) || !std::isfinite(g->y[v])) reorder = false;   // keep the caller's order
+            x0 = std::min(x0, g->x[v]); x1 = std::max(x1, g->x[v]);
+            y0 = std::min(y0, g->y[v]); y1 = std::max(y1, g->y[v]);
+        }
+    }
+    if (reorder) {
+        const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+        auto spread = [](u32 a) {   // 16 bits -> even bit positions
+            a &= 0xFFFFu;
+            a = (a | (a << 8)) & 0x00FF00FFu;
+            a = (a | (a << 4)) & 0x0F0F0F0Fu;
+            a = (a | (a << 2)) & 0x33333333u;
+            a = (a | (a << 1)) & 0x55555555u;
+            return a;
+        };
+        std::vector<u64> key(s->n);
+#pragma omp parallel for schedule(static)
+        for (long long v = 0; v < (long long)s->n; ++v) {
+            const u32 qx = (u32)((g->x[v] - x0) * sx), qy = (u32)((g->y[v] - y0) * sy);
+            key[v] = ((u64)(spread(qx) | (spread(qy) << 1)) << 32) | (u64)v;
+        }
+        std::sort(key.begin(), key.end());
+        s->perm.resize(s->n);
+        s->inv.resize(s->n);
+        for (u32 i = 0; i < s->n; ++i) {
+            s->perm[i] = (u32)key[i];
+            s->inv[s->perm[i]] = i;
+        }
+    }
+    s->callerRow.assign(g->row_offsets, g->row_offsets + s->n + 1);
     // ---- CSR upload: interleaved {dst, w}, pinned staging, separate streams ----
     if ((rc = s->dalloc(&s->d_row, (u64)s->n + 1)) || (rc = s->dalloc(&s->d_adj, s->m)) ||
         (rc = s->dalloc(&s->d_vrec, (u64)s->n + 1)))
         return cleanupFail(rc);
     {
-        const size_t bRow = (size_t)(s->n + 1) * 4, bAdj = (size_t)s->m * 8;
+        const size_t bRow = (size_t)(s->n + 1) * 4, bAdj = (size_t)s->m * 8, bOrig = (size_t)s->n * 4;
         char* stage = nullptr;
-        if (hipHostMalloc((void**)&stage, bRow + bAdj, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void**)&stage, bRow + bAdj + bOrig, hipHostMallocDefault) != hipSuccess)
             return cleanupFail(fail(PF_ERR_NO_MEMORY, "pinned staging"));
-        std::memcpy(stage, g->row_offsets, bRow);
+        u32* row = (u32*)stage;
         uint2* adj = (uint2*)(stage + bRow);
-        std::vector<u32> col, w;
-        const u32* cp = g->col_indices;
-        const u32* wp = g->weights;
-        if (g->flags & PF_GRAPH_SORT_ROWS) {
-            col.assign(g->col_indices, g->col_indices + g->m);
-            w.assign(g->weights, g->weights + g->m);
-            sortRows(s->n, g->row_offsets, col.data(), w.data());
-            cp = col.data();
-            wp = w.data();
-        }
+        u32* orig = (u32*)(stage + bRow + bAdj);
+        if (reorder) {
+            row[0] = 0;
+            for (u32 i = 0; i < s->n; ++i) {
+                const u32 v = s->perm[i];
+                row[i + 1] = row[i] + (g->row_offsets[v + 1] - g->row_offsets[v]);
+                orig[i] = v;
+            }
+#pragma omp parallel
+            {
+                std::vector<u64> tmp;
+#pragma omp for schedule(dynamic, 4096)
+                for (long long i = 0; i < (long long)s->n; ++i) {
+                    const u32 v = s->perm[i], b = g->row_offsets[v], e = g->row_offsets[v + 1];
+                    tmp.resize(e - b);
+                    for (u32 k = b; k < e; ++k)
+                        tmp[k - b] = ((u64)std::max(1u, g->weights[k]) << 32) | s->inv[g->col_indices[k]];
+                    std::sort(tmp.begin(), tmp.end());   // rows sorted by (w, dst) in the new numbering
+                    for (u32 k = 0; k < e - b; ++k) adj[row[i] + k] = make_uint2((u32)tmp[k], (u32)(tmp[k] >> 32));
+                }
+            }
+        } else {
+            std::memcpy(row, g->row_offsets, bRow);
+            std::vector<u32> col, w;
+            const u32* cp = g->col_indices;
+            const u32* wp = g->weights;
+            if (g->flags & PF_GRAPH_SORT_ROWS) {
+                col.assign(g->col_indices, g->col_indices + g->m);
+                w.assign(g->weights, g->weights + g->m);
+                sortRows(s->n, g->row_offsets, col.data(), w.data());
+                cp = col.data();
+                wp = w.data();
+            }
 #pragma omp parallel for schedule(static)
-        for (long long k = 0; k < (long long)s->m; ++k) adj[k] = make_uint2(cp[k], wp[k] < 1u ? 1u : wp[k]);
+            for (long long k = 0; k < (long long)s->m; ++k) adj[k] = make_uint2(cp[k], wp[k] < 1u ? 1u : wp[k]);
+            for (u32 i = 0; i < s->n; ++i) orig[i] = i;
+        }
+        u32* d_orig = nullptr;
+        if ((rc = s->dalloc(&d_orig, s->n))) { hipHostFree(stage); return cleanupFail(rc); }
         hipStream_t st2;
-        if (hipStreamCreate(&st2) != hipSuccess) { hipHostFree(stage); return cleanupFail(fail(PF_ERR_HIP, "stream")); }
-        hipMemcpyAsync(s->d_row, stage, bRow, hipMemcpyHostToDevice, s->s);
-        hipMemcpyAsync(s->d_adj, stage + bRow, bAdj, hipMemcpyHostToDevice, st2);
+        if (hipStreamCreate(&st2) != hipSuccess) { hipHostFree(stage); hipFree(d_orig); return cleanupFail(fail(PF_ERR_HIP, "stream")); }
+        hipMemcpyAsync(s->d_row, row, bRow, hipMemcpyHostToDevice, s->s);
+        hipMemcpyAsync(d_orig, orig, bOrig, hipMemcpyHostToDevice, s->s);
+        hipMemcpyAsync(s->d_adj, adj, bAdj, hipMemcpyHostToDevice, st2);
         hipError_t e1 = hipStreamSynchronize(s->s), e2 = hipStreamSynchronize(st2);
         hipStreamDestroy(st2);
         hipHostFree(stage);
-        if (e1 != hipSuccess || e2 != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "CSR upload"));
+        if (e1 != hipSuccess || e2 != hipSuccess) { hipFree(d_orig); return cleanupFail(fail(PF_ERR_HIP, "CSR upload")); }
+        // vertex records valid from the start (every arc light); Delta runs recompute the split
+        hipLaunchKernelGGL(vrecInitKernel, dim3(divUp(s->n + 1, 256)), dim3(256), 0, s->s, s->d_row, d_orig, s->n,
+                           s->d_vrec);
+        hipError_t e3 = hipGetLastError();
+        if (e3 == hipSuccess) e3 = hipStreamSynchronize(s->s);
+        hipFree(d_orig);
+        s->bytes -= (u64)s->n * 4;
+        if (e3 != hipSuccess) return cleanupFail(fail(PF_ERR_HIP, "vertex records"));
+    }
+    if (reorder) {
+        if ((rc = s->dalloc(&s->d_inv, s->n))) return cleanupFail(rc);
+        if (hipMemcpy(s->d_inv, s->inv.data(), (size_t)s->n * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanupFail(fail(PF_ERR_HIP, "permutation upload"));
     }
     if (coords) {
         bool integral = true;
@@ -736,23 +845,21 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
         }
         s->intCoords = integral;
         std::vector<double2> xyd(s->n);
-        for (u32 v = 0; v < s->n; ++v) xyd[v] = make_double2(g->x[v], g->y[v]);
+        for (u32 i = 0; i < s->n; ++i) {
+            const u32 v = reorder ? s->perm[i] : i;
+            xyd[i] = make_double2(g->x[v], g->y[v]);
+        }
         if ((rc = s->dalloc(&s->d_xyd, s->n))) return cleanupFail(rc);
         if (hipMemcpy(s->d_xyd, xyd.data(), (size_t)s->n * 16, hipMemcpyHostToDevice) != hipSuccess)
             return cleanupFail(fail(PF_ERR_HIP, "coords upload"));
         if (integral) {
             std::vector<int2> xyi(s->n);
-            for (u32 v = 0; v < s->n; ++v) xyi[v] = make_int2((int)g->x[v], (int)g->y[v]);
+            for (u32 i = 0; i < s->n; ++i) xyi[i] = make_int2((int)xyd[i].x, (int)xyd[i].y);
             if ((rc = s->dalloc(&s->d_xyi, s->n))) return cleanupFail(rc);
             if (hipMemcpy(s->d_xyi, xyi.data(), (size_t)s->n * 8, hipMemcpyHostToDevice) != hipSuccess)
                 return cleanupFail(fail(PF_ERR_HIP, "coords upload"));
         }
     }
-    // vertex records valid from the start (every arc light); Delta runs recompute the split
-    hipLaunchKernelGGL(lightEndKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, s->d_row, s->d_adj, s->n,
-                       0xFFFFFFFFu, s->d_vrec);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->s) != hipSuccess)
-        return cleanupFail(fail(PF_ERR_HIP, "vertex records"));
     s->lightDelta = 0xFFFFFFFFu;
     s->uploadMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
@@ -788,7 +895,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
@@ -962,10 +1069,12 @@ int pf_serve_admit(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, co
             if (slots[j] == slots[i]) return fail(PF_ERR_INVALID, "slot admitted twice");
     }
     u32* h = s->h_admit;
-    std::memcpy(h, slots, (size_t)count * 4);
-    std::memcpy(h + s->K, srcs, (size_t)count * 4);
-    if (dsts) std::memcpy(h + 2 * s->K, dsts, (size_t)count * 4);
-    else std::memset(h + 2 * s->K, 0, (size_t)count * 4);
+    PF_HIP(hipStreamSynchronize(s->s));   // the pinned staging may still feed an earlier copy
+    for (u32 i = 0; i < count; ++i) {
+        h[i] = slots[i];
+        h[s->K + i] = toIn(s, srcs[i]);
+        h[2 * s->K + i] = dsts ? toIn(s, dsts[i]) : 0u;
+    }
     PF_HIP(hipMemcpyAsync(s->d_admit, h, 3ull * s->K * 4, hipMemcpyHostToDevice, s->s));
     const u64 total = (u64)count * s->n;
     hipLaunchKernelGGL(slotInitKernel, dim3(std::min<u32>(divUp(total, 256), 8192u)), dim3(256), 0, s->s, s->d_state,
@@ -1028,12 +1137,14 @@ int pf_serve_astar_results(pf_solver* s, const uint32_t* slots, const uint32_t* 
         s->pathCap = need;
     }
     u32* h = s->h_admit;
+    PF_HIP(hipStreamSynchronize(s->s));
     std::memcpy(h, slots, (size_t)count * 4);
-    std::memcpy(h + s->K, srcs, (size_t)count * 4);
+    std::memcpy(h + s->K, srcs, (size_t)count * 4);      // caller's ids: the chase maps them
     std::memcpy(h + 2 * s->K, dsts, (size_t)count * 4);
     PF_HIP(hipMemcpyAsync(s->d_admit, h, 3ull * s->K * 4, hipMemcpyHostToDevice, s->s));
-    hipLaunchKernelGGL(chaseSlotsKernel, dim3(divUp(count, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_admit,
-                       s->d_admit + s->K, s->d_admit + 2 * s->K, count, s->d_paths, path_cap, s->d_lens, s->d_tdist);
+    hipLaunchKernelGGL(chaseSlotsKernel, dim3(divUp(count, 64)), dim3(64), 0, s->s, s->d_state, s->n, s->d_inv,
+                       s->d_admit, s->d_admit + s->K, s->d_admit + 2 * s->K, count, s->d_paths, path_cap, s->d_lens,
+                       s->d_tdist);
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(target_dists, s->d_tdist, (size_t)count * 4, hipMemcpyDeviceToHost, s->s));
     PF_HIP(hipMemcpyAsync(path_lens, s->d_lens, (size_t)count * 4, hipMemcpyDeviceToHost, s->s));
@@ -1050,9 +1161,10 @@ int pf_serve_sssp_result(pf_solver* s, uint32_t slot, uint32_t* dist, uint32_t* 
     std::vector<u64> hst(s->n);
     PF_HIP(hipMemcpyAsync(hst.data(), s->d_state + (u64)slot * s->n, (size_t)s->n * 8, hipMemcpyDeviceToHost, s->s));
     PF_HIP(hipStreamSynchronize(s->s));
-    for (u32 v = 0; v < s->n; ++v) {
-        if (dist) dist[v] = (u32)(hst[v] >> 32);
-        if (parent) parent[v] = (u32)hst[v];
+    for (u32 i = 0; i < s->n; ++i) {
+        const u32 v = s->perm.empty() ? i : s->perm[i];
+        if (dist) dist[v] = (u32)(hst[i] >> 32);
+        if (parent) parent[v] = (u32)hst[i];
     }
     return PF_OK;
 }
@@ -1086,9 +1198,13 @@ int pf_debug_light_end(pf_solver* s, uint32_t delta, uint32_t* light_end) {
     s->lightDelta = 0;
     rc = ensureLightEnd(s, delta);
     if (rc) return rc;
-    std::vector<uint2> vr(s->n);
-    PF_HIP(hipMemcpy(vr.data(), s->d_vrec, (size_t)s->n * 8, hipMemcpyDeviceToHost));
-    for (u32 v = 0; v < s->n; ++v) light_end[v] = vr[v].y;
+    std::vector<uint4> vr(s->n + 1);
+    PF_HIP(hipMemcpy(vr.data(), s->d_vrec, (size_t)(s->n + 1) * 16, hipMemcpyDeviceToHost));
+    // in the caller's CSR: light end = caller's row start + light arcs of the row
+    for (u32 i = 0; i < s->n; ++i) {
+        const u32 v = vr[i].z;
+        light_end[v] = s->callerRow[v] + (vr[i].y - vr[i].x);
+    }
     return PF_OK;
 }
 
@@ -1104,12 +1220,14 @@ int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h) {
     run.hkind = (u32)kind;
     u32* d_h = nullptr;
     PF_HIP(hipMalloc((void**)&d_h, (size_t)s->n * 4));
-    hipLaunchKernelGGL(heuristicKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, run, target, d_h);
+    hipLaunchKernelGGL(heuristicKernel, dim3(divUp(s->n, 256)), dim3(256), 0, s->s, run, toIn(s, target), d_h);
+    std::vector<u32> hi(s->n);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(h, d_h, (size_t)s->n * 4, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), d_h, (size_t)s->n * 4, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipStreamSynchronize(s->s);
     hipFree(d_h);
     if (e != hipSuccess) return fail(PF_ERR_HIP, hipGetErrorString(e));
+    for (u32 i = 0; i < s->n; ++i) h[s->perm.empty() ? i : s->perm[i]] = hi[i];
     return PF_OK;
 }
 
@@ -1158,8 +1276,20 @@ int pf_debug_sorted_rows(pf_solver* s, uint32_t* col, uint32_t* w) {
     int rc = setDev(s);
     if (rc) return rc;
     std::vector<uint2> adj(s->m);
+    std::vector<u32> row(s->n + 1);
     PF_HIP(hipMemcpy(adj.data(), s->d_adj, (size_t)s->m * 8, hipMemcpyDeviceToHost));
-    for (u32 k = 0; k < s->m; ++k) { col[k] = adj[k].x; w[k] = adj[k].y; }
+    PF_HIP(hipMemcpy(row.data(), s->d_row, (size_t)(s->n + 1) * 4, hipMemcpyDeviceToHost));
+    // rows in the caller's order and ids, each sorted by (w, dst) as the engine holds it
+    std::vector<u64> tmp;
+    for (u32 v = 0; v < s->n; ++v) {
+        const u32 i = toIn(s, v), b = row[i], e = row[i + 1];
+        tmp.clear();
+        for (u32 k = b; k < e; ++k)
+            tmp.push_back(((u64)adj[k].y << 32) | (s->perm.empty() ? adj[k].x : s->perm[adj[k].x]));
+        if (!s->perm.empty()) std::sort(tmp.begin(), tmp.end());
+        const u32 o = s->callerRow[v];
+        for (u32 k = 0; k < e - b; ++k) { col[o + k] = (u32)tmp[k]; w[o + k] = (u32)(tmp[k] >> 32); }
+    }
     return PF_OK;
 }
 

@@ -69,8 +69,11 @@ static_assert(sizeof(Ctl) == 128, "control block is eight 16-byte words");
 struct DevGraph {
     const u32* row;       // row_offsets [n+1]
     const uint2* adj;     // {dst, w} [m], each row sorted by (w, dst)
-    const uint2* vrec;    // [n+1] {row_offsets[u], first heavy arc of row u for the run's delta}:
-                          // one 8-byte record per vertex, so expanding u touches one line
+    const uint4* vrec;    // [n+1] {row_offsets[u], first heavy arc of row u for the run's delta,
+                          // caller's id of u, 0}: one 16-byte record per vertex, so expanding u
+                          // touches one line.  Vertices are numbered internally (Morton order for
+                          // graphs with coordinates); the packed state keeps the caller's parent
+                          // id, so equal-distance ties resolve exactly as in the caller's numbering.
     const int2* xyi;      // integer coordinates [n] (FLOOR_EUCLID) or null
     const double2* xyd;   // double coordinates [n] (SCALED) or null
     u32 n, m;
@@ -179,13 +182,29 @@ __device__ __forceinline__ u32 waveAggInc(u32* ctr) {
 __device__ __forceinline__ void pushQ(u32* data, u32* ctr, u32 item) { stA(&data[waveAggInc(ctr)], item); }
 
 // Read-then-atomicOr dedup bit (src/gpu_kernels.cuh:124-129).
+// Mask words change across the steps of one persistent launch, whose vector L1
+// is never invalidated: the pre-check reads the device-coherent value (a stale
+// "set" would drop a push).
 __device__ __forceinline__ bool testAndSetBit(u32* mask, u32 i) {
     u32* wp = mask + (i >> 5);
     const u32 bit = 1u << (i & 31u);
-    if (*wp & bit) return false;
+    if (ldA(wp) & bit) return false;
     return (atomicOr(wp, bit) & bit) == 0u;
 }
 __device__ __forceinline__ void clearBit(u32* mask, u32 i) { atomicAnd(mask + (i >> 5), ~(1u << (i & 31u))); }
+// Clears the bit and returns the old mask word (the atomic has been performed
+// at L2 once the value is back).
+__device__ __forceinline__ u32 clearBitRet(u32* mask, u32 i) {
+    return atomicAnd(mask + (i >> 5), ~(1u << (i & 31u)));
+}
+// Zero that becomes available only when v has arrived.  Added to an address it
+// orders the dependent access after the memory operation that produced v,
+// without a fence or cache maintenance (a register dependency, no memory access).
+__device__ __forceinline__ u32 zeroAfter(u32 v) {
+    u32 z;
+    asm volatile("v_and_b32 %0, 0, %1" : "=v"(z) : "v"(v));
+    return z;
+}
 
 // Exact floor(sqrt(a)) for a < 2^53.
 __device__ __forceinline__ u32 isqrtFloor(u64 a) {
@@ -269,7 +288,7 @@ struct Outs {
 
 // Per-vertex context of an expanded frontier element.
 struct Item {
-    u32 u, du, k, beg, end;
+    u32 u, p, du, k, beg, end;   // p: caller's id of u, written as the parent
     u64 ub;       // A*: dist[target] of the slot (prune key > ub)
     u64 bnd;      // bucket bound of the slot (key < bnd -> near)
     Target t;
@@ -287,7 +306,7 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
         if (key > it.ub) return;               // strict: ties with dist[target] stay explored
     }
     const u32 gv = it.k * r.g.n + v;
-    if (relaxAtomic<AM>(&r.state[gv], (u32)nd, it.u, ls)) {
+    if (relaxAtomic<AM>(&r.state[gv], (u32)nd, it.p, ls)) {
         ls.updates++;
         if (o.flags) {
             stA(&o.flags[gv], 1u);
@@ -321,7 +340,7 @@ __device__ __forceinline__ u32 waveInclusiveSum(u32 x) {
 struct GatherSlot {
     u32 excl, u, du, k, beg;
     u32 ubLo, ubHi, bLo, bHi;
-    u32 pad;
+    u32 p;
 };
 static_assert(sizeof(GatherSlot) == 40, "gather slot");
 
@@ -342,14 +361,17 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
     for (u32 base0 = blockIdx.x * bs; base0 < size; base0 += gridDim.x * bs) {   // block-uniform
         const u32 idx = base0 + threadIdx.x;
         Item it;
-        it.u = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
+        it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
         it.ub = NOKEY; it.bnd = NOKEY;
         if (idx < size) {
             const u32 gi = ldA(&in[idx]);
-            if (inMask) clearBit(inMask, gi);          // may re-enter the frontier later
+            // The bit is cleared before the distance is read, and the read waits for the
+            // clear: a relaxation that lowers dist[gi] after the read then finds the bit
+            // clear and pushes gi again (it tests the bit only after its atomicMin).
+            const u32 z = inMask ? zeroAfter(clearBitRet(inMask, gi)) : 0u;
             it.k = slotOf(r, gi);
             it.u = gi - it.k * n;
-            it.du = distOf(ldA(&r.state[gi]));         // the latest distance: expanded with its final value
+            it.du = distOf(ldA(&r.state[gi + z]));     // the latest distance: expanded with its final value
             bool active = it.du != INF;
             if constexpr (ASTAR) {
                 const u32 t = r.target[it.k];
@@ -358,8 +380,9 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                 if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
             }
             if (active) {
-                const uint2 va = r.g.vrec[it.u], vb = r.g.vrec[it.u + 1];
-                const u32 r0 = va.x, le = va.y, r1 = vb.x;
+                const uint4 va = r.g.vrec[it.u];
+                const u32 r0 = va.x, le = va.y, r1 = r.g.vrec[it.u + 1].x;
+                it.p = va.z;
                 // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
                 if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &r.ctl->cnt[C_R], gi);
                 if (sel == 0) { it.beg = r0; it.end = r1; }
@@ -395,6 +418,7 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                 ballot &= ballot - 1;
                 Item wi;
                 wi.u = __shfl(it.u, leader, 64);
+                wi.p = __shfl(it.p, leader, 64);
                 wi.du = __shfl(it.du, leader, 64);
                 wi.k = __shfl(it.k, leader, 64);
                 wi.beg = __shfl(it.beg, leader, 64);
@@ -419,7 +443,7 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
             me.excl = incl - deg; me.u = it.u; me.du = it.du; me.k = it.k; me.beg = it.beg;
             me.ubLo = (u32)it.ub; me.ubHi = (u32)(it.ub >> 32);
             me.bLo = (u32)it.bnd; me.bHi = (u32)(it.bnd >> 32);
-            me.pad = 0;
+            me.p = it.p;
             gs[lane] = me;
             __syncthreads();
             {
@@ -433,7 +457,7 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                         }
                         const GatherSlot ow = gs[lo];
                         Item gi;
-                        gi.u = ow.u; gi.du = ow.du; gi.k = ow.k;
+                        gi.u = ow.u; gi.p = ow.p; gi.du = ow.du; gi.k = ow.k;
                         gi.ub = ((u64)ow.ubHi << 32) | ow.ubLo;
                         gi.bnd = ((u64)ow.bHi << 32) | ow.bLo;
                         if constexpr (ASTAR) gi.t = loadTarget(r, r.target[gi.k]);
@@ -466,8 +490,9 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
             if (active && (u64)it.du + heur(r, t, u) > it.ub) active = false;
         }
         if (!active) continue;
+        const uint4 va = r.g.vrec[u];
         const u32 r1 = r.g.vrec[u + 1].x;
-        for (u32 e = r.g.vrec[u].x; e < r1; ++e) {
+        for (u32 e = va.x; e < r1; ++e) {
             const uint2 a = r.g.adj[e];
             const u64 nd = (u64)it.du + a.y;
             ls.edges++;
@@ -475,7 +500,7 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
             if constexpr (ASTAR) {
                 if (nd + heur(r, t, a.x) > it.ub) continue;
             }
-            if (relaxAtomic<AM>(&r.state[a.x], (u32)nd, u, ls)) {
+            if (relaxAtomic<AM>(&r.state[a.x], (u32)nd, va.z, ls)) {
                 ls.updates++;
                 stA(&actOut[a.x], (u8)1);
                 if (ldAgent(changed) == 0u) stAgent(changed, 1u);
@@ -513,7 +538,7 @@ __device__ void farMin(const Run& r, const u32* far, u32 size) {
                 key = o2 < key ? o2 : key;
             }
             if (laneId() == 0 && key != NOKEY) atomicMin(&r.minKey[0], key);
-        } else if (key != NOKEY && key < r.minKey[k]) {
+        } else if (key != NOKEY && key < ldA(&r.minKey[k])) {
             atomicMin(&r.minKey[k], key);
         }
     }
@@ -1018,17 +1043,13 @@ __global__ void epochKernel(Run r, u32 steps) {
     r.bar[BAR_GEN] = 0ull;
 }
 
-// Paths of selected slots, followed on the device (reconstructPath,
-// src/cpu_sssp.cpp:124-135): one thread per listed slot.
-__global__ void chaseSlotsKernel(const u64* state, u32 n, const u32* slots, const u32* srcs, const u32* targets,
-                                 u32 count, u32* paths, u32 cap, u32* lens, u32* tdist) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const u64* st = state + (u64)slots[i] * n;
-    const u32 s = srcs[i], t = targets[i];
-    u32* p = paths + (u64)i * cap;
-    const u32 d = distOf(st[t]);
-    tdist[i] = d;
+// Path from s to t in one slot's state row, in the caller's ids (parents are
+// stored in the caller's ids; inv maps them to state indices, null = identity).
+// reconstructPath (src/cpu_sssp.cpp:124-135); returns the length, 0 if t is
+// unreached or the chain does not end at s within cap vertices.
+__device__ u32 chasePath(const u64* st, u32 n, const u32* inv, u32 s, u32 t, u32* p, u32 cap, u32* tdist) {
+    const u32 d = distOf(st[inv ? inv[t] : t]);
+    *tdist = d;
     u32 len = 0;
     if (d != INF) {
         u32 v = t;
@@ -1036,17 +1057,33 @@ __global__ void chaseSlotsKernel(const u64* state, u32 n, const u32* slots, cons
         for (u32 steps = 0; steps <= n && len < cap; ++steps) {
             p[len++] = v;
             if (v == s) { ok = true; break; }
-            v = (u32)st[v];
+            v = (u32)st[inv ? inv[v] : v];
             if (v == NOPAR || v >= n) break;
         }
         if (!ok) len = 0;
         for (u32 j = 0; j < len / 2; ++j) { const u32 x = p[j]; p[j] = p[len - 1 - j]; p[len - 1 - j] = x; }
     }
-    lens[i] = len;
+    return len;
 }
 
-// lightEndKernel (src/gpu_kernels.cuh:361-371).
-__global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, uint2* vrec) {
+// Paths of selected slots (serving), one thread per listed slot; caller's ids.
+__global__ void chaseSlotsKernel(const u64* state, u32 n, const u32* inv, const u32* slots, const u32* srcs,
+                                 const u32* targets, u32 count, u32* paths, u32 cap, u32* lens, u32* tdist) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    lens[i] = chasePath(state + (u64)slots[i] * n, n, inv, srcs[i], targets[i], paths + (u64)i * cap, cap, &tdist[i]);
+}
+
+// Vertex records at creation: row start, every arc light, caller's id.
+__global__ void vrecInitKernel(const u32* row, const u32* orig, u32 n, uint4* vrec) {
+    const u32 u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) vrec[u] = make_uint4(row[u], row[u + 1], orig[u], 0u);
+    if (u == n) vrec[n] = make_uint4(row[n], row[n], NOPAR, 0u);
+}
+
+// lightEndKernel (src/gpu_kernels.cuh:361-371): first heavy arc of each row
+// into the vertex records (fields .x/.z/.w unchanged).
+__global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delta, uint4* vrec) {
     const u32 u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u < n) {
         u32 lo = row[u], hi = row[u + 1];
@@ -1054,9 +1091,8 @@ __global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delt
             const u32 mid = lo + ((hi - lo) >> 1);
             if (adj[mid].y <= delta) lo = mid + 1; else hi = mid;
         }
-        vrec[u] = make_uint2(row[u], lo);
+        vrec[u].y = lo;
     }
-    if (u == 0) vrec[n] = make_uint2(row[n], row[n]);
 }
 
 // heuristicKernel (src/gpu_kernels.cuh:355-358) -- test hook; the engine
@@ -1066,31 +1102,12 @@ __global__ void heuristicKernel(Run r, u32 target, u32* h) {
     if (v < r.g.n) h[v] = heur(r, loadTarget(r, target), v);
 }
 
-// Path of each slot, followed from its target on the device (reconstructPath,
-// src/cpu_sssp.cpp:124-135): one thread per slot.
-__global__ void chaseKernel(const u64* state, u32 n, const u32* srcs, const u32* targets, u32 K, u32* paths,
-                            u32 cap, u32* lens, u32* tdist) {
+// Path of each slot k, followed from its target on the device; caller's ids.
+__global__ void chaseKernel(const u64* state, u32 n, const u32* inv, const u32* srcs, const u32* targets, u32 K,
+                            u32* paths, u32 cap, u32* lens, u32* tdist) {
     const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    const u64* st = state + (u64)k * n;
-    const u32 s = srcs[k], t = targets[k];
-    u32* p = paths + (u64)k * cap;
-    const u32 d = distOf(st[t]);
-    tdist[k] = d;
-    u32 len = 0;
-    if (d != INF) {
-        u32 v = t;
-        bool ok = false;
-        for (u32 steps = 0; steps <= n && len < cap; ++steps) {
-            p[len++] = v;
-            if (v == s) { ok = true; break; }
-            v = (u32)st[v];
-            if (v == NOPAR || v >= n) break;
-        }
-        if (!ok) len = 0;
-        for (u32 i = 0; i < len / 2; ++i) { const u32 x = p[i]; p[i] = p[len - 1 - i]; p[len - 1 - i] = x; }
-    }
-    lens[k] = len;
+    lens[k] = chasePath(state + (u64)k * n, n, inv, srcs[k], targets[k], paths + (u64)k * cap, cap, &tdist[k]);
 }
 
 // Reached vertices and their out-arcs over K slots (TEPS numerator).

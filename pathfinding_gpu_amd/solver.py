@@ -124,12 +124,16 @@ def warmup() -> None:
     N.check(N.engine().pf_warmup())
 
 
-def _describe(g: CSRGraph) -> N.PfGraph:
+GRAPH_SORT_ROWS = 1   # the engine sorts each row by (w, dst)
+GRAPH_NO_REORDER = 2  # keep the caller's vertex numbering inside the engine
+
+
+def _describe(g: CSRGraph, flags: int = 0) -> N.PfGraph:
     d = N.PfGraph()
     d.n, d.m = g.n, g.m
     d.row_offsets, d.col_indices, d.weights = N.ptr(g.row_offsets), N.ptr(g.col_indices), N.ptr(g.weights)
     d.x, d.y = N.ptr(g.x, C.c_double), N.ptr(g.y, C.c_double)
-    d.h_scale, d.avg_w, d.avg_deg, d.flags = g.h_scale, g.avg_w, g.avg_deg, 0
+    d.h_scale, d.avg_w, d.avg_deg, d.flags = g.h_scale, g.avg_w, g.avg_deg, flags
     return d
 
 
@@ -144,11 +148,14 @@ def transfer_study(g: CSRGraph, reps: int = 3) -> dict:
 class GpuSolver:
     """Graph resident in HBM + working buffers for up to max_batch slots."""
 
-    def __init__(self, g: CSRGraph, max_batch: int = 1):
+    def __init__(self, g: CSRGraph, max_batch: int = 1, reorder: bool = True):
+        """reorder=False keeps the caller's vertex numbering inside the engine (graphs with
+        coordinates are otherwise renumbered in Morton order; results are identical)."""
         self.g = g   # kept alive: the engine copied it, the reference keeps a reference
         self._lib = N.engine()
         h = C.c_void_p()
-        N.check(self._lib.pf_solver_create(C.byref(_describe(g)), max_batch, C.byref(h)))
+        flags = 0 if reorder else GRAPH_NO_REORDER
+        N.check(self._lib.pf_solver_create(C.byref(_describe(g, flags)), max_batch, C.byref(h)))
         self._h = h
         self.n = g.n
         self.max_batch = max_batch

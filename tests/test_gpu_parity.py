@@ -259,3 +259,55 @@ def test_reach_stats(orc, grid100, sgrid100):
     d, _ = oracle_sssp(orc, grid100, 0)
     want_arcs, want_reached = orc.reachable_arcs(grid100.row_offsets, d)
     assert (reached, arcs) == (want_reached, want_arcs)
+
+
+def test_internal_renumbering_is_invisible(orc, grid100, sgrid100):
+    """Graphs with coordinates are renumbered internally (Morton order); the caller's
+    numbering kept inside the engine gives the same dist, parent and paths."""
+    with P.GpuSolver(grid100, max_batch=8, reorder=False) as plain:
+        for algo in ALGOS:
+            a = sgrid100.sssp(777, P.GpuOptions(algo=algo))
+            b = plain.sssp(777, P.GpuOptions(algo=algo))
+            assert np.array_equal(a.dist, b.dist) and np.array_equal(a.parent, b.parent), algo.name
+        assert_sssp_exact(orc, grid100, 777, b, "no-reorder delta")
+        srcs = [0, 5, 999, 4321, 9999, 17, 2500, 7000]
+        ms_a = sgrid100.multi_source(srcs, P.GpuOptions(algo=GA.DeltaStep, drive=DR.Persistent))
+        ms_b = plain.multi_source(srcs, P.GpuOptions(algo=GA.DeltaStep, drive=DR.Persistent))
+        assert np.array_equal(ms_a.dist, ms_b.dist) and np.array_equal(ms_a.parent, ms_b.parent)
+        dsts = [9999, 0, 1, 5000, 123, 8888, 42, 3]
+        for kind in (HK.Scaled, HK.FloorEuclid):
+            o = P.GpuOptions(algo=GA.DeltaStep, heuristic=kind)
+            ra = sgrid100.astar_batch(srcs, dsts, o)
+            rb = plain.astar_batch(srcs, dsts, o)
+            assert np.array_equal(ra["dist"], rb["dist"])
+            assert all(np.array_equal(x, y) for x, y in zip(ra["paths"], rb["paths"]))
+            assert all(len(x) > 1 for x in ra["paths"][:3])
+        ha = sgrid100.debug_heuristic(HK.FloorEuclid, 4242)
+        hb = plain.debug_heuristic(HK.FloorEuclid, 4242)
+        assert np.array_equal(ha, hb)
+
+
+@pytest.fixture(scope="module")
+def grid1000():
+    return P.grid_road(1000, 1000, 7)
+
+
+def test_persistent_full_size_grid(orc, grid1000):
+    """1000x1000 road grid (1M vertices), persistent drive: Delta from 1x to 32x the
+    default, single queries and a 16-source batch, every mapping of the single query.
+    Steps of one persistent launch share no cache invalidation, so masks, minima and
+    distances written in one step and read in a later one are all exercised here."""
+    g = grid1000
+    with P.GpuSolver(g, max_batch=16) as sv:
+        auto = sv.auto_delta()
+        for mult in (1, 32):
+            for mp in MAPS:
+                o = P.GpuOptions(drive=DR.Persistent, delta=auto * mult, mapping=mp)
+                r = sv.sssp(0, o)
+                assert_sssp_exact(orc, g, 0, r, f"grid1000 single delta {auto * mult} {mp.name}")
+        srcs = np.random.default_rng(5).choice(g.n, 16, replace=False)
+        want = orc.multi_source(g.row_offsets, g.col_indices, g.weights, srcs, 0)[0]
+        for mult in (1, 32):
+            o = P.GpuOptions(drive=DR.Persistent, delta=auto * mult, mapping=MP.Hybrid)
+            r = sv.multi_source(srcs, o)
+            assert np.array_equal(r.dist, want.reshape(16, g.n)), f"grid1000 batch delta {auto * mult}"
