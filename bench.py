@@ -34,6 +34,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TUNE_DB = os.path.join(ROOT, "pathfinding_gpu_amd", "data", "autotune.json")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stepkernel_grid1000_k64_hbm_traffic.md")
+
+
+def measured_traffic(opts: dict):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3 counter summary,
+    if it was measured with exactly these launch options (else None)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic: "):
+                rec = json.loads(line[len("traffic: "):])
+                if rec.get("options") == opts:
+                    return rec["fetch_bytes"] + rec["write_bytes"]
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -56,6 +72,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="grid1000", choices=["grid1000", "rmat20", "union512"])
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-parameter autotuner")
+    ap.add_argument("--tune-db", default=TUNE_DB, help="autotuner results (JSON, per graph and batch size)")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -114,6 +131,10 @@ def main():
         sv.multi_source(srcs, opt(d, mp), copy_back=False)
         return min(sv.multi_source(srcs, opt(d, mp), copy_back=False).ms for _ in range(2))
 
+    def tf_batch_o(o):
+        sv.multi_source(srcs, o, copy_back=False)
+        return min(sv.multi_source(srcs, o, copy_back=False).ms for _ in range(2))
+
     log("[bench] choosing Delta")
     pick1 = tuner.choose(sv, g, 1, tf_single)
     pickK = tuner.choose(sv, g, K, tf_batch)
@@ -132,11 +153,20 @@ def main():
         def tfo(o):
             return sv.multi_source(srcs, o, copy_back=False).ms
 
-        tuned = autotune(g, K, pickK["delta"], info["cu_count"], tfo, passes=1, reps=1, cache=None)
+        tuned = autotune(g, K, pickK["delta"], info["cu_count"], tfo, passes=1, reps=1, cache=args.tune_db)
         cand = tuned.options(info["cu_count"], delta=pickK["delta"])
-        if tuned.ms < tf_batch(pickK["delta"], pickK["mapping"]):
+        if tf_batch_o(cand) < tf_batch(pickK["delta"], pickK["mapping"]):
             learnedK = cand
-        log(f"[bench] autotuned: {tuned}")
+        log(f"[bench] autotuned (batch): {tuned}")
+
+        def tfo1(o):
+            return sv.sssp(src0, o, copy_back=False).ms
+
+        tuned1 = autotune(g, 1, pick1["delta"], info["cu_count"], tfo1, passes=1, reps=2, cache=args.tune_db)
+        cand1 = tuned1.options(info["cu_count"], delta=pick1["delta"])
+        if min(tfo1(cand1) for _ in range(2)) < tf_single(pick1["delta"], pick1["mapping"]):
+            learned1 = cand1
+        log(f"[bench] autotuned (single): {tuned1}")
 
     # ---- warm-up, then timed steps: the learned batched run ----
     for _ in range(args.warmup):
@@ -165,6 +195,10 @@ def main():
     bytes_launch = 16 * reached_k + 16 * arcs_k
     loop_ms = r_step.stats.loop_ms
     achieved = bytes_launch / (loop_ms * 1e-3) / 1e9
+    timed_opts = {"delta": int(learnedK.delta), "mapping": int(learnedK.mapping), "block_size": int(learnedK.block_size),
+                  "wave_cutoff": int(learnedK.wave_cutoff), "block_cutoff": int(learnedK.block_cutoff),
+                  "grid_blocks": int(learnedK.grid_blocks)}
+    traffic = measured_traffic(timed_opts) if args.workload == "grid1000" and K == 64 else None
 
     out = {
         "metric": "GTEPS, learned-Delta batched (K=64 SSSP per launch) vs plain default-Delta single query; "
@@ -197,6 +231,8 @@ def main():
         "speedup_vs_plain": round(gteps / gteps_plain, 3),
         "host_driven_reference_style_ms": round(ms_host, 3),
         "learned_single_ms": round(ms_l1, 3),
+        "learned_single_options": {"delta": int(learned1.delta), "mapping": int(learned1.mapping),
+                                   "block_size": int(learned1.block_size), "grid_blocks": int(learned1.grid_blocks)},
         "learned_single_gteps": round(arcs1 / (ms_l1 * 1e-3) / 1e9, 5),
         "learned_single_speedup": round(ms_plain / ms_l1, 3),
         "step_stats": {"steps": r_step.stats.steps, "buckets": r_step.stats.buckets,
@@ -204,8 +240,10 @@ def main():
                        "atomics_per_update": round(r_step.stats.atomics_issued / max(1, r_step.stats.updates), 4),
                        "event_ms": round(r_step.ms, 3), "loop_ms": round(loop_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "bytes_per_launch": int(bytes_launch), "kernel": "stepKernel (persistent, K-source batch)"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else int(traffic),
+                     "bytes_per_launch": int(bytes_launch), "kernel": "stepKernel (persistent, K-source batch)",
+                     "launch_options": timed_opts},
     }
 
     if QA:

@@ -1,43 +1,151 @@
-"""Fixed workload for hardware-counter profiles (rocprofv3 --pmc ... -- python3 -m pathfinding_gpu_amd.profile_run).
+"""Fixed workload for hardware-counter profiles, and the summary of those profiles.
 
-Runs, in order: one streaming copy of 1 GiB (counter calibration: 1 GiB read, 1 GiB
-written), then `--steps` batched runs of K=64 single-source Delta-stepping on the
-1000x1000 grid road graph (seed 7; vertex 0 + 63 random sources, persistent
-kernel, Delta = 1019, hybrid mapping) -- the benchmark's step -- and prints the
-reach statistics that give the step kernel's algorithmic bytes.
+Run (one counter per rocprofv3 run, on the GPU):
+
+    rocprofv3 --pmc FETCH_SIZE --kernel-trace -d OUT/FETCH_SIZE -o run -- \
+        python3 -m pathfinding_gpu_amd.profile_run [--options JSON]
+    rocprofv3 --pmc WRITE_SIZE --kernel-trace -d OUT/WRITE_SIZE -o run -- \
+        python3 -m pathfinding_gpu_amd.profile_run [--options JSON]
+
+runs, in order: one streaming copy of 1 GiB (counter calibration: 1 GiB read,
+1 GiB written), then `--steps` batched runs of K=64 single-source Delta-stepping
+on the 1000x1000 grid road graph (seed 7; vertex 0 + 63 random sources, the
+benchmark's step), with the persistent kernel and the launch options given as
+JSON (the benchmark prints the ones it timed), and prints the reach statistics
+that give the step kernel's algorithmic bytes.
+
+Kernel-trace summary of any run (per kernel and launch shape):
+
+    python3 -m pathfinding_gpu_amd.profile_run --trace-summary OUT/bench_results.db
+
+Summarize (on any machine, from the two results databases):
+
+    python3 -m pathfinding_gpu_amd.profile_run --summarize OUT [--options JSON] > summary.md
+
+scales each counter by what it reads for the copy kernel (bytes per KiB of
+counter), and writes a markdown table plus one machine-readable line
+`traffic: {...}` that the benchmark reads to fill roofline.traffic when it
+timed the same launch options.
 """
 import argparse
 import ctypes as C
 import json
+import os
+import sqlite3
 
 import numpy as np
 
-from . import _native as N
-from .graph import grid_road
-from .solver import Drive, GpuOptions, GpuSolver, Mapping
+BATCH = 64
+COPY_BYTES = 1 << 30
+
+
+def workload(batch: int = BATCH):
+    from .graph import grid_road
+    g = grid_road(1000, 1000, 7)
+    rng = np.random.default_rng(2026)
+    cand = np.flatnonzero(np.diff(g.row_offsets) > 0)
+    srcs = np.concatenate([[0], rng.choice(cand[cand != 0], size=batch - 1, replace=False)]).astype(np.uint32)
+    return g, srcs
+
+
+def options_from(d: dict):
+    from .solver import Drive, GpuOptions, Mapping
+    return GpuOptions(drive=Drive.Persistent, delta=int(d["delta"]), mapping=Mapping(int(d.get("mapping", 2))),
+                      block_size=int(d.get("block_size", 256)), wave_cutoff=int(d.get("wave_cutoff", 0)),
+                      block_cutoff=int(d.get("block_cutoff", 0)), grid_blocks=int(d.get("grid_blocks", 0)))
+
+
+def run(a):
+    from . import _native as N
+    from .solver import GpuSolver
+    ms = C.c_double()
+    N.check(N.engine().pf_debug_copy(COPY_BYTES, 1, C.byref(ms)))
+    g, srcs = workload(a.batch)
+    sv = GpuSolver(g, max_batch=a.batch)
+    opts = json.loads(a.options) if a.options else {"delta": sv.auto_delta(), "mapping": 2}
+    o = options_from(opts)
+    rs = [sv.multi_source(srcs, o, copy_back=False) for _ in range(a.steps)]
+    reached, arcs = sv.last_reach()
+    print(json.dumps({"options": opts, "copy_ms_1GiB": ms.value, "copy_GBps": 2 * COPY_BYTES / (ms.value * 1e-3) / 1e9,
+                      "reached": reached, "arcs": arcs, "bytes_alg_per_launch": 16 * reached + 16 * arcs,
+                      "loop_ms": [r.stats.loop_ms for r in rs], "edges_scanned": rs[-1].stats.edges_scanned,
+                      "updates": rs[-1].stats.updates, "atomics": rs[-1].stats.atomics_issued}))
+    sv.close()
+
+
+def _rows(db: str):
+    con = sqlite3.connect(db)
+    try:
+        return con.execute("select dispatch_id, kernel_name, grid_size, workgroup_size, value, duration "
+                           "from counters_collection order by dispatch_id").fetchall()
+    finally:
+        con.close()
+
+
+def summarize(a):
+    res = {c: _rows(os.path.join(a.summarize, c, "run_results.db")) for c in ("FETCH_SIZE", "WRITE_SIZE")}
+
+    def scale(c, true_bytes):
+        cp = [r[4] for r in res[c] if "copyKernel" in r[1]]
+        return true_bytes / (cp[-1] * 1024.0), cp[-1]
+
+    fs, fraw = scale("FETCH_SIZE", COPY_BYTES)
+    ws, wraw = scale("WRITE_SIZE", COPY_BYTES)
+    fr = [r for r in res["FETCH_SIZE"] if "stepKernel" in r[1]]
+    wr = [r for r in res["WRITE_SIZE"] if "stepKernel" in r[1]]
+    print("| kernel | workgroups x size | FETCH_SIZE KiB | WRITE_SIZE KiB | fetched bytes (scaled) "
+          "| written bytes (scaled) | duration us |")
+    print("|---|---|---|---|---|---|---|")
+    fb, wb = [], []
+    for x, y in zip(fr, wr):
+        fb.append(x[4] * 1024 * fs)
+        wb.append(y[4] * 1024 * ws)
+        print(f"| {x[1].split('(')[0]} | {x[2] // x[3]} x {x[3]} | {x[4]:.0f} | {y[4]:.0f} | {fb[-1]:.4g} | "
+              f"{wb[-1]:.4g} | {x[5] / 1000:.0f} |")
+    print()
+    print(f"Calibration: the 1 GiB copy kernel reads {fraw:.0f} KiB of FETCH_SIZE (x{fs:.4f} to bytes) and "
+          f"{wraw:.0f} KiB of WRITE_SIZE (x{ws:.4f}).")
+    rec = {"options": json.loads(a.options) if a.options else None, "fetch_bytes": float(np.mean(fb)),
+           "write_bytes": float(np.mean(wb)), "fetch_scale": fs, "write_scale": ws}
+    print()
+    print("traffic: " + json.dumps(rec))
+
+
+def trace_summary(db: str):
+    """Per kernel and launch shape of a rocprofv3 --kernel-trace results database: calls, total,
+    average, min and max duration (the figures of rocprofv3 --stats)."""
+    import collections
+    con = sqlite3.connect(db)
+    try:
+        rows = con.execute("select name, grid_x, workgroup_x, duration from kernels").fetchall()
+    finally:
+        con.close()
+    agg = collections.defaultdict(list)
+    for name, gx, wx, d in rows:
+        agg[(name.split("(")[0], gx // max(1, wx), wx)].append(d / 1000.0)
+    tot = sum(sum(v) for v in agg.values())
+    print("| kernel | workgroups | wg size | calls | total ms | avg us | min us | max us | % |")
+    print("|---|---|---|---|---|---|---|---|---|")
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| {k[0] or '(unnamed)'} | {k[1]} | {k[2]} | {len(v)} | {sum(v) / 1e3:.3f} | {sum(v) / len(v):.2f} | "
+              f"{min(v):.2f} | {max(v):.2f} | {100 * sum(v) / tot:.1f} |")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--mapping", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--options", default="", help="JSON launch options of the batched run "
+                                                  "(delta, mapping, block_size, wave_cutoff, block_cutoff, grid_blocks)")
+    ap.add_argument("--summarize", default="", help="directory holding FETCH_SIZE/ and WRITE_SIZE/ results")
+    ap.add_argument("--trace-summary", default="", help="rocprofv3 --kernel-trace results database")
     a = ap.parse_args()
-    ms = C.c_double()
-    N.check(N.engine().pf_debug_copy(1 << 30, 1, C.byref(ms)))
-    g = grid_road(1000, 1000, 7)
-    rng = np.random.default_rng(2026)
-    cand = np.flatnonzero(np.diff(g.row_offsets) > 0)
-    srcs = np.concatenate([[0], rng.choice(cand[cand != 0], size=a.batch - 1, replace=False)]).astype(np.uint32)
-    sv = GpuSolver(g, max_batch=a.batch)
-    o = GpuOptions(drive=Drive.Persistent, delta=sv.auto_delta(), mapping=Mapping(a.mapping))
-    rs = [sv.multi_source(srcs, o, copy_back=False) for _ in range(a.steps)]
-    reached, arcs = sv.last_reach()
-    print(json.dumps({"copy_ms_1GiB": ms.value, "copy_GBps": 2 * 2**30 / (ms.value * 1e-3) / 1e9,
-                      "reached": reached, "arcs": arcs, "bytes_alg_per_launch": 16 * reached + 16 * arcs,
-                      "loop_ms": [r.stats.loop_ms for r in rs], "edges_scanned": rs[-1].stats.edges_scanned,
-                      "updates": rs[-1].stats.updates, "atomics": rs[-1].stats.atomics_issued}))
-    sv.close()
+    if a.trace_summary:
+        trace_summary(a.trace_summary)
+    elif a.summarize:
+        summarize(a)
+    else:
+        run(a)
 
 
 if __name__ == "__main__":
