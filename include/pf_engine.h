@@ -41,8 +41,11 @@ typedef enum { PF_COMPACT_WAVE_ATOMIC = 0, PF_COMPACT_PREFIX_SCAN = 1 } pf_compa
  * drives it from the host, gpu_sssp.cu:390-455). */
 typedef enum { PF_DRIVE_HOST = 0, PF_DRIVE_GRAPH = 1, PF_DRIVE_PERSISTENT = 2 } pf_drive;
 /* A* heuristic.  SCALED = the reference's heuristicValue (include/heuristic.h:19-25);
- * FLOOR_EUCLID = floor(Euclidean distance) on integer coordinates. */
-typedef enum { PF_H_NONE = 0, PF_H_SCALED = 1, PF_H_FLOOR_EUCLID = 2 } pf_heuristic;
+ * FLOOR_EUCLID = floor(Euclidean distance) on integer coordinates; ALT = landmark
+ * lower bound max_L |d_L(t) - d_L(v)| (d_L(t) - d_L(v) on a non-symmetric graph)
+ * from the landmarks of pf_solver_build_landmarks (no reference counterpart).
+ * Every heuristic here is consistent, so the canonical path does not depend on it. */
+typedef enum { PF_H_NONE = 0, PF_H_SCALED = 1, PF_H_FLOOR_EUCLID = 2, PF_H_ALT = 3 } pf_heuristic;
 
 /* GpuOptions (gpu_sssp.h:42-52) plus the MI355X knobs. */
 typedef struct {
@@ -122,6 +125,16 @@ uint64_t pf_solver_device_bytes(const pf_solver* s);      /* deviceBytes() gpu_s
 uint32_t pf_solver_auto_delta(const pf_solver* s);        /* autoDelta() gpu_sssp.h:107, gpu_sssp.cu:202-206 */
 uint32_t pf_solver_n(const pf_solver* s);
 uint32_t pf_solver_max_batch(const pf_solver* s);
+
+/* Landmarks for the ALT heuristic (no reference counterpart; A* serving).
+ * count: 4, 8, 12 or 16.  ids: caller's vertex ids of the landmarks, or NULL to
+ * choose them farthest-first (the first one farthest from vertex 0, each next one
+ * farthest from those chosen).  Runs one SSSP per landmark on the device and keeps
+ * count u32 distances per vertex resident (4*count*n bytes).  chosen_out (count
+ * entries, or NULL) receives the landmark ids; *symmetric_out (or NULL) whether
+ * every arc has a reverse arc of equal weight (then |d_L(t) - d_L(v)| bounds). */
+int pf_solver_build_landmarks(pf_solver* s, uint32_t count, const uint32_t* ids, uint32_t* chosen_out,
+                              uint32_t* symmetric_out);
 
 /* GpuSolver::sssp (gpu_sssp.cu:545).  dist/parent: n entries each, or NULL
  * (= copyBack false). */

@@ -63,6 +63,7 @@ ENGINE_SYMBOLS = {
     "pf_solver_upload_ms": (C.c_double, [C.c_void_p]),
     "pf_solver_device_bytes": (C.c_uint64, [C.c_void_p]),
     "pf_solver_auto_delta": (C.c_uint32, [C.c_void_p]),
+    "pf_solver_build_landmarks": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, u32p]),
     "pf_solver_n": (C.c_uint32, [C.c_void_p]),
     "pf_solver_max_batch": (C.c_uint32, [C.c_void_p]),
     "pf_sssp": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p, C.POINTER(PfStats)]),

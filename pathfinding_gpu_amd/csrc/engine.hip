@@ -83,6 +83,8 @@ struct pf_solver {
     uint2* d_adj = nullptr;
     uint4* d_vrec = nullptr;   // {row_offsets[u], light end of row u, original id, 0} [n+1]
     u32* d_inv = nullptr;      // original id -> internal id (null: identity)
+    u32* d_lm = nullptr;       // landmark distances [n][lmCount], internal order (ALT heuristic)
+    u32 lmCount = 0, lmSym = 0;
     std::vector<u32> perm, inv; // internal -> original, original -> internal (empty: identity)
     std::vector<u32> callerRow; // the caller's row_offsets (test hooks report in the caller's CSR)
     int2* d_xyi = nullptr;
@@ -166,6 +168,21 @@ void sortRows(u32 n, const u32* row, u32* col, u32* w) {
         std::sort(tmp.begin(), tmp.end());
         for (u32 k = b; k < e; ++k) { w[k] = (u32)(tmp[k - b] >> 32); col[k] = (u32)tmp[k - b]; }
     }
+}
+
+// Heuristic kind of an A* run for option value h (PF_H_*).
+int pickHeuristic(const pf_solver* s, int h, u32* hk) {
+    *hk = HK_NONE;
+    if (h == PF_H_FLOOR_EUCLID) {
+        if (!s->intCoords) return fail(PF_ERR_INVALID, "FLOOR_EUCLID heuristic needs integer coordinates");
+        *hk = HK_FLOOR_EUCLID;
+    } else if (h == PF_H_SCALED && s->hasCoords && s->hScale > 0.0) {
+        *hk = HK_SCALED;
+    } else if (h == PF_H_ALT) {
+        if (!s->lmCount) return fail(PF_ERR_STATE, "ALT heuristic: pf_solver_build_landmarks first");
+        *hk = HK_ALT;
+    }
+    return PF_OK;
 }
 
 // caller's vertex id -> internal id
@@ -293,6 +310,9 @@ int ensureFlags(pf_solver* s, u32 tileItems) {
 Run baseRun(const pf_solver* s) {
     Run run{};
     run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.lm = s->d_lm;
+    run.lmCount = s->lmCount;
+    run.lmSym = s->lmSym;
     run.state = s->d_state;
     run.ctl = s->d_ctl;
     run.stats = s->d_stats;
@@ -339,14 +359,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         if (rc) return rc;
     }
     u32 hk = HK_NONE;
-    if (spec.astar) {
-        if (o.heuristic == PF_H_FLOOR_EUCLID) {
-            if (!s->intCoords) return fail(PF_ERR_INVALID, "FLOOR_EUCLID heuristic needs integer coordinates");
-            hk = HK_FLOOR_EUCLID;
-        } else if (o.heuristic == PF_H_SCALED && s->hasCoords && s->hScale > 0.0) {
-            hk = HK_SCALED;
-        }
-    }
+    if (spec.astar && (rc = pickHeuristic(s, o.heuristic, &hk))) return rc;
     const bool scanMode = o.algo == PF_ALGO_FRONTIER && o.compaction == PF_COMPACT_PREFIX_SCAN;
     const u32 bs = (u32)o.block_size;
     const u32 tileItems = bs * 16;
@@ -378,6 +391,9 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
 
     Run run{};
     run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.lm = s->d_lm;
+    run.lmCount = s->lmCount;
+    run.lmSym = s->lmSym;
     run.state = s->d_state;
     run.ctl = s->d_ctl;
     run.stats = s->d_stats;
@@ -895,7 +911,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
@@ -918,6 +934,95 @@ uint64_t pf_solver_device_bytes(const pf_solver* s) { return s->bytes; }
 uint32_t pf_solver_auto_delta(const pf_solver* s) { return autoDelta(s); }
 uint32_t pf_solver_n(const pf_solver* s) { return s->n; }
 uint32_t pf_solver_max_batch(const pf_solver* s) { return s->K; }
+
+int pf_solver_build_landmarks(pf_solver* s, uint32_t count, const uint32_t* ids, uint32_t* chosen_out,
+                              uint32_t* symmetric_out) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (count == 0 || count > 16 || count % 4) return fail(PF_ERR_INVALID, "landmark count must be 4, 8, 12 or 16");
+    if (s->serving) return fail(PF_ERR_STATE, "pf_serve_end first");
+    if (ids)
+        for (u32 j = 0; j < count; ++j)
+            if (ids[j] >= s->n) return fail(PF_ERR_INVALID, "landmark out of range");
+    // symmetric: every arc (u, v, w) has a reverse arc (v, u, w)
+    u32 sym = 1;
+    {
+        std::vector<uint2> adj(s->m);
+        std::vector<u32> row(s->n + 1);
+        PF_HIP(hipMemcpy(adj.data(), s->d_adj, (size_t)s->m * 8, hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpy(row.data(), s->d_row, (size_t)(s->n + 1) * 4, hipMemcpyDeviceToHost));
+        std::vector<u64> key(s->m);
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (long long u = 0; u < (long long)s->n; ++u) {
+            for (u32 e = row[u]; e < row[u + 1]; ++e) key[e] = ((u64)adj[e].y << 32) | adj[e].x;
+            std::sort(key.begin() + row[u], key.begin() + row[u + 1]);
+        }
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
+        for (long long u = 0; u < (long long)s->n; ++u) {
+            for (u32 e = row[u]; e < row[u + 1] && !bad; ++e) {
+                const u32 v = adj[e].x;
+                const u64 want = ((u64)adj[e].y << 32) | (u64)u;
+                if (!std::binary_search(key.begin() + row[v], key.begin() + row[v + 1], want)) bad = 1;
+            }
+        }
+        sym = bad ? 0u : 1u;
+    }
+    if (s->d_lm) {
+        hipFree(s->d_lm);
+        s->bytes -= (u64)s->n * s->lmCount * 4;
+        s->d_lm = nullptr;
+    }
+    s->lmCount = 0;
+    if ((rc = s->dalloc(&s->d_lm, (u64)s->n * count))) return rc;
+    pf_options o;
+    pf_default_options(&o);
+    o.drive = PF_DRIVE_PERSISTENT;
+    RunSpec spec;
+    spec.nseeds = 1;
+    spec.k = 1;
+    spec.o = o;
+    std::vector<u64> st(s->n);
+    std::vector<u32> mind;
+    auto farthest = [&]() -> u32 {   // caller id of the vertex farthest from those in mind (finite only)
+        u32 best = 0, bd = 0;
+        for (u32 i = 0; i < s->n; ++i)
+            if (mind[i] != INF && mind[i] > bd) { bd = mind[i]; best = i; }
+        return s->perm.empty() ? best : s->perm[best];
+    };
+    auto runFrom = [&](u32 src) -> int {
+        spec.seeds = &src;
+        int r2 = doRun(s, spec, nullptr);
+        if (r2) return r2;
+        PF_HIP(hipMemcpy(st.data(), s->d_state, (size_t)s->n * 8, hipMemcpyDeviceToHost));
+        return PF_OK;
+    };
+    u32 next = 0;
+    if (!ids) {   // start: the vertex farthest from vertex 0
+        if ((rc = runFrom(0))) return rc;
+        mind.resize(s->n);
+        for (u32 i = 0; i < s->n; ++i) mind[i] = (u32)(st[i] >> 32);
+        next = farthest();
+        std::fill(mind.begin(), mind.end(), INF);
+    }
+    for (u32 j = 0; j < count; ++j) {
+        const u32 lid = ids ? ids[j] : next;
+        if ((rc = runFrom(lid))) return rc;
+        hipLaunchKernelGGL(lmColumnKernel, dim3(std::min<u32>(divUp(s->n, 256), 8192u)), dim3(256), 0, s->s,
+                           s->d_state, s->n, count, j, s->d_lm);
+        PF_HIP(hipGetLastError());
+        if (chosen_out) chosen_out[j] = lid;
+        if (!ids) {
+            for (u32 i = 0; i < s->n; ++i) mind[i] = std::min(mind[i], (u32)(st[i] >> 32));
+            next = farthest();
+        }
+    }
+    PF_HIP(hipStreamSynchronize(s->s));
+    s->lmCount = count;
+    s->lmSym = sym;
+    if (symmetric_out) *symmetric_out = sym;
+    return PF_OK;
+}
 
 int pf_sssp(pf_solver* s, uint32_t src, const pf_options* o, uint32_t* dist, uint32_t* parent, pf_stats* st) {
     if (!s || !o) return fail(PF_ERR_INVALID, "null argument");
@@ -1008,14 +1113,7 @@ int pf_serve_begin(pf_solver* s, const pf_options* opt, int astar) {
     rc = ensureLightEnd(s, delta);
     if (rc) return rc;
     u32 hk = HK_NONE;
-    if (astar) {
-        if (o.heuristic == PF_H_FLOOR_EUCLID) {
-            if (!s->intCoords) return fail(PF_ERR_INVALID, "FLOOR_EUCLID heuristic needs integer coordinates");
-            hk = HK_FLOOR_EUCLID;
-        } else if (o.heuristic == PF_H_SCALED && s->hasCoords && s->hScale > 0.0) {
-            hk = HK_SCALED;
-        }
-    }
+    if (astar && (rc = pickHeuristic(s, o.heuristic, &hk))) return rc;
     if (!s->d_admit) {
         if ((rc = s->dalloc(&s->d_admit, 3ull * s->K))) return rc;
         PF_HIP(hipHostMalloc((void**)&s->h_admit, 3ull * s->K * 4, hipHostMallocDefault));
@@ -1214,8 +1312,12 @@ int pf_debug_heuristic(pf_solver* s, int kind, uint32_t target, uint32_t* h) {
     if (target >= s->n) return fail(PF_ERR_INVALID, "target out of range");
     if (kind == PF_H_FLOOR_EUCLID && !s->intCoords) return fail(PF_ERR_INVALID, "no integer coordinates");
     if (kind == PF_H_SCALED && !s->hasCoords) return fail(PF_ERR_INVALID, "no coordinates");
+    if (kind == PF_H_ALT && !s->lmCount) return fail(PF_ERR_STATE, "no landmarks");
     Run run{};
     run.g = DevGraph{s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->n, s->m};
+    run.lm = s->d_lm;
+    run.lmCount = s->lmCount;
+    run.lmSym = s->lmSym;
     run.hScale = s->hScale;
     run.hkind = (u32)kind;
     u32* d_h = nullptr;

@@ -35,7 +35,7 @@ constexpr u64 NOKEY = ~0ull;
 enum { AM_MIN_RBA = 0, AM_CAS_RBA = 1, AM_MIN_NOCHECK = 2 };
 enum { MAP_THREAD = 0, MAP_WAVE = 1, MAP_HYBRID = 2 };
 enum { ALGO_NAIVE = 0, ALGO_FRONTIER = 1, ALGO_DELTA = 2 };
-enum { HK_NONE = 0, HK_SCALED = 1, HK_FLOOR_EUCLID = 2 };
+enum { HK_NONE = 0, HK_SCALED = 1, HK_FLOOR_EUCLID = 2, HK_ALT = 3 };
 
 enum Phase : u32 {
     PH_LIGHT = 0,        // relax light edges of the bucket, mark R
@@ -109,6 +109,9 @@ struct Run {
     u32 total;                   // K*n
     unsigned long long* bar;     // persistent grid barrier: [0] ticket, [BAR_GEN] released barriers
     unsigned long long spinTicks;  // wall-clock bound of one barrier wait
+    const u32* lm;               // landmark distances [n][lmCount] (HK_ALT), internal vertex order
+    u32 lmCount;                 // landmarks per vertex, a multiple of 4, <= 16
+    u32 lmSym;                   // graph is symmetric: distances to a landmark equal distances from it
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -220,12 +223,38 @@ __device__ __forceinline__ u32 isqrtFloor(u64 a) {
 struct Target {
     int2 ti;
     double2 td;
+    u32 id;
 };
 __device__ __forceinline__ Target loadTarget(const Run& r, u32 t) {
     Target x;
     x.ti = r.g.xyi ? r.g.xyi[t] : make_int2(0, 0);
     x.td = r.g.xyd ? r.g.xyd[t] : make_double2(0.0, 0.0);
+    x.id = t;
     return x;
+}
+
+// Landmark (ALT) lower bound on dist(v, t): for each landmark L with distances
+// d_L from it, d_L(t) - d_L(v) <= dist(v, t) (triangle inequality), and on a
+// symmetric graph also d_L(v) - d_L(t).  The maximum over landmarks is
+// admissible and consistent, so strict pruning keeps every vertex of every
+// shortest path and the canonical path is unchanged.  Landmarks that do not
+// reach both vertices give no bound.
+__device__ __forceinline__ u32 heurAlt(const Run& r, u32 t, u32 v) {
+    const uint4* lt = reinterpret_cast<const uint4*>(r.lm + (u64)t * r.lmCount);
+    const uint4* lv = reinterpret_cast<const uint4*>(r.lm + (u64)v * r.lmCount);
+    u32 best = 0;
+    for (u32 j = 0; j < r.lmCount / 4; ++j) {
+        const uint4 a = lt[j], b = lv[j];
+        const u32 av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (av[q] == INF || bv[q] == INF) continue;
+            u32 d = av[q] > bv[q] ? av[q] - bv[q] : 0u;
+            if (r.lmSym && bv[q] > av[q]) d = bv[q] - av[q];
+            best = d > best ? d : best;
+        }
+    }
+    return best;
 }
 __device__ __forceinline__ u32 heurScaled(double2 p, double2 t, double scale) {
 #pragma clang fp contract(off)
@@ -243,6 +272,7 @@ __device__ __forceinline__ u32 heur(const Run& r, const Target& t, u32 v) {
         return isqrtFloor((u64)(dx * dx) + (u64)(dy * dy));
     }
     if (r.hkind == HK_SCALED) return heurScaled(r.g.xyd[v], t.td, r.hScale);
+    if (r.hkind == HK_ALT) return heurAlt(r, t.id, v);
     return 0u;
 }
 
@@ -1093,6 +1123,12 @@ __global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delt
         }
         vrec[u].y = lo;
     }
+}
+
+// One landmark's distances into column j of the [n][count] landmark table.
+__global__ void lmColumnKernel(const u64* state, u32 n, u32 count, u32 j, u32* lm) {
+    for (u32 v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        lm[(u64)v * count + j] = distOf(state[v]);
 }
 
 // heuristicKernel (src/gpu_kernels.cuh:355-358) -- test hook; the engine

@@ -54,13 +54,15 @@ class SlotEngine:
     """One resident graph in serving mode."""
 
     def __init__(self, g: CSRGraph, slots: int, opt: GpuOptions, astar: bool, keep_sssp: bool = True,
-                 path_cap: int = 16384):
+                 path_cap: int = 16384, landmarks: int = 0):
+        """landmarks > 0: build that many landmarks before serving (for Heuristic.ALT)."""
         self.g = g
         self.astar = astar
         self.K = slots
         self.sv = GpuSolver(g, max_batch=slots)
         self._lib = self.sv._lib
         self.opt = opt
+        self.landmarks = self.sv.build_landmarks(landmarks)[0] if landmarks else []
         N.check(self._lib.pf_serve_begin(self.sv._h, C.byref(opt.to_c()), 1 if astar else 0))
         self.free: List[int] = list(range(slots))[::-1]
         self.busy: Dict[int, Request] = {}

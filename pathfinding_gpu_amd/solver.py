@@ -52,6 +52,7 @@ class Heuristic(IntEnum):
     None_ = 0
     Scaled = 1
     FloorEuclid = 2
+    ALT = 3          # landmark lower bound (GpuSolver.build_landmarks first)
 
 
 @dataclass
@@ -182,6 +183,15 @@ class GpuSolver:
 
     def auto_delta(self) -> int:
         return self._lib.pf_solver_auto_delta(self._h)
+
+    def build_landmarks(self, count: int = 8, ids: Optional[Sequence[int]] = None):
+        """Landmarks for Heuristic.ALT: count in {4, 8, 12, 16}; ids None = farthest-first.
+        Returns (landmark ids, symmetric graph)."""
+        chosen = np.empty(count, np.uint32)
+        sym = C.c_uint32()
+        idv = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        N.check(self._lib.pf_solver_build_landmarks(self._h, count, N.ptr(idv), N.ptr(chosen), C.byref(sym)))
+        return chosen.tolist(), bool(sym.value)
 
     def sssp(self, src: int, opt: GpuOptions = GpuOptions(), copy_back: bool = True) -> GpuRunResult:
         st = N.PfStats()
