@@ -1,0 +1,214 @@
+// CPU emulation driver (test infrastructure): runs the step kernel of
+// pathfinding_gpu_amd/csrc/pf_device.hpp the way the engine drives it -- one step per launch
+// (workgroups one after another, or up to 4 at once) or one persistent launch of up to 4
+// concurrent workgroups -- on a generated graph, with device buffers filled with garbage as
+// hipMalloc leaves them, and checks dist and parent (A*: target distance and canonical path)
+// against the oracle.  Prints "ok: ..." or "FAIL: ...".
+//   emu g=grid|rmat size= algo=naive|frontier|scan|delta map= am= bs= grid= K= kmax= delta=
+//       pers= astar= lm= sym= src= yield= fill= seed=
+#include "shim.h"
+#include "dev_emu_0.hpp"   // one copy of the device code per concurrent workgroup: the copies'
+#include "dev_emu_1.hpp"   // function-scope statics stand for each workgroup's LDS
+#include "dev_emu_2.hpp"
+#include "dev_emu_3.hpp"
+#include <algorithm>
+#include <string>
+#include <map>
+extern "C" {
+#include "pf_graph.h"
+#include "oracle.h"
+}
+using namespace pf0;
+
+#define V3(NS, A, AM, R, P) \
+    if (map == 0) NS::stepKernel<A, AM, 0>(R, P); else if (map == 1) NS::stepKernel<A, AM, 1>(R, P); else NS::stepKernel<A, AM, 2>(R, P);
+#define V9(NS, A, R, P) \
+    if (am == 0) { V3(NS, A, 0, R, P) } else if (am == 1) { V3(NS, A, 1, R, P) } else { V3(NS, A, 2, R, P) }
+#define VARIANTS(NS, R, P) \
+    if (astar) { V9(NS, true, R, P) } else { V9(NS, false, R, P) }
+
+static void stepB(u32 b, Run& r, u32 persistent, int am, int map, u32 astar) {
+    switch (b & 3) {
+        case 0: { VARIANTS(pf0, r, persistent) break; }
+        case 1: { auto& q = *reinterpret_cast<pf1::Run*>(&r); VARIANTS(pf1, q, persistent) break; }
+        case 2: { auto& q = *reinterpret_cast<pf2::Run*>(&r); VARIANTS(pf2, q, persistent) break; }
+        default: { auto& q = *reinterpret_cast<pf3::Run*>(&r); VARIANTS(pf3, q, persistent) break; }
+    }
+}
+
+// device buffers: hipMalloc does not clear memory; fill with a garbage pattern unless the
+// engine clears the buffer itself (zero=true)
+static int g_fill = 0xA5;
+template <class T> T* dalloc(size_t n, bool zero = false) {
+    T* p = (T*)std::malloc(std::max<size_t>(n, 1) * sizeof(T));
+    std::memset(p, zero ? 0 : g_fill, std::max<size_t>(n, 1) * sizeof(T));
+    return p;
+}
+static u64 maskWords(u64 bits) { return (bits + 31) / 32; }
+
+int main(int argc, char** argv) {
+    std::map<std::string, std::string> A;
+    for (int i = 1; i < argc; ++i) { std::string a = argv[i]; auto p = a.find('='); A[a.substr(0, p)] = a.substr(p + 1); }
+    auto gi_ = [&](const char* k, long d) { return A.count(k) ? std::stol(A[k]) : d; };
+    std::string gk = A.count("g") ? A["g"] : "grid";
+    const u32 size = gi_("size", 10);
+    std::string algo = A.count("algo") ? A["algo"] : "delta";
+    const int map = gi_("map", 2), am = gi_("am", 0);
+    const u32 bs = gi_("bs", 64), grid = gi_("grid", 1), K = gi_("K", 1);
+    const u32 Kmax = std::max<u32>(K, gi_("kmax", K));
+    u32 delta = gi_("delta", 0);
+    const int persistent = gi_("pers", 0);
+    const u32 astar = gi_("astar", 0);
+    const long src0 = gi_("src", -1);
+    emuYieldEvery = (uint32_t)gi_("yield", 0);
+    g_fill = (int)gi_("fill", 0xA5);
+    emuRng ^= (uint64_t)gi_("seed", 1) * 0x9E3779B97F4A7C15ull;
+    std::vector<u32> src, dst, w;
+    std::vector<double> x, y;
+    u32 n;
+    std::vector<u32> row, col, wt;
+    u32 m = 0;
+    if (gk == "grid" || gk == "rmat") {
+        if (gk == "grid") {
+            n = size * size;
+            const u64 mm = pf_grid_road_arcs(size, size);
+            src.resize(mm); dst.resize(mm); w.resize(mm); x.resize(n); y.resize(n);
+            pf_gen_grid_road(size, size, 42, 30, 1.0, 1.3, src.data(), dst.data(), w.data(), x.data(), y.data());
+        } else {
+            n = 1u << size;
+            const u64 mm = 16ull << size;
+            src.resize(mm); dst.resize(mm); w.resize(mm);
+            pf_gen_rmat(size, 16, 0.57, 0.19, 0.19, 100, 11, src.data(), dst.data(), w.data());
+        }
+        row.resize(n + 1); col.resize(src.size()); wt.resize(src.size());
+        pf_build_csr(n, src.size(), src.data(), dst.data(), w.data(), row.data(), col.data(), wt.data(), &m);
+    } else {
+        u32 hasxy = 0;
+        pf_csr_peek(gk.c_str(), &n, &m, &hasxy);
+        row.resize(n + 1); col.resize(m); wt.resize(m);
+        if (hasxy) { x.resize(n); y.resize(n); }
+        pf_csr_load(gk.c_str(), row.data(), col.data(), wt.data(), hasxy ? x.data() : nullptr, hasxy ? y.data() : nullptr);
+    }
+    // engine layout: rows sorted by (w, dst), w >= 1, vertex records
+    uint2* adj = dalloc<uint2>(m);
+    for (u32 u = 0; u < n; ++u) {
+        std::vector<u64> t;
+        for (u32 e = row[u]; e < row[u + 1]; ++e) t.push_back(((u64)std::max(1u, wt[e]) << 32) | col[e]);
+        std::sort(t.begin(), t.end());
+        for (u32 k = 0; k < t.size(); ++k) adj[row[u] + k] = make_uint2((u32)t[k], (u32)(t[k] >> 32));
+    }
+    u32* drow = dalloc<u32>(n + 1);
+    std::copy(row.begin(), row.end(), drow);
+    uint4* vrec = dalloc<uint4>(n + 1);
+    double avgW = 0; for (u32 e = 0; e < m; ++e) avgW += adj[e].y; avgW /= std::max(1u, m);
+    if (!delta) delta = (u32)std::max(1.0, std::round(32.0 * avgW / ((double)m / n)));
+    for (u32 u = 0; u < n; ++u) {
+        u32 le = row[u];
+        if (algo == "delta") while (le < row[u + 1] && adj[le].y <= delta) ++le;
+        else le = row[u + 1];
+        vrec[u] = make_uint4(row[u], le, u, 0);
+    }
+    vrec[n] = make_uint4(row[n], row[n], NOPAR, 0);
+    int2* xyi = nullptr; double2* xyd = nullptr;
+    if (!x.empty()) {
+        xyi = dalloc<int2>(n); xyd = dalloc<double2>(n);
+        for (u32 v = 0; v < n; ++v) { xyi[v] = make_int2((int)x[v], (int)y[v]); xyd[v] = make_double2(x[v], y[v]); }
+    }
+    const u64 Kn = (u64)K * n, KnMax = (u64)Kmax * n;
+    Run r{};
+    r.g = DevGraph{drow, adj, vrec, xyi, xyd, n, m};
+    r.state = dalloc<u64>(KnMax);
+    r.ctl = dalloc<Ctl>(1, true);
+    r.stats = dalloc<unsigned long long>(5, true);
+    for (int i = 0; i < 2; ++i) {
+        r.nearQ[i] = dalloc<u32>(KnMax); r.nearMask[i] = dalloc<u32>(maskWords(KnMax));
+        r.farQ[i] = dalloc<u32>(KnMax); r.act[i] = dalloc<u8>(n);
+    }
+    r.farMask = dalloc<u32>(maskWords(KnMax)); r.rQ = dalloc<u32>(KnMax); r.rMask = dalloc<u32>(maskWords(KnMax));
+    const bool scan = algo == "scan";
+    const u32 tileItems = bs * 16;
+    r.flags = scan ? dalloc<u32>(KnMax, true) : nullptr;
+    r.tileSums = dalloc<u32>((KnMax + tileItems - 1) / tileItems);
+    r.bound = dalloc<u64>(Kmax); r.newBound = dalloc<u64>(Kmax); r.minKey = dalloc<unsigned long long>(Kmax);
+    u32* target = dalloc<u32>(Kmax);
+    r.target = astar ? target : nullptr;
+    r.slotDone = dalloc<u32>(Kmax);
+    r.hScale = 0.0;
+    r.invN = 1.0 / n;
+    r.K = K; r.delta = delta; r.hkind = (astar && xyi) ? HK_FLOOR_EUCLID : HK_NONE;
+    const u32 L = gi_("lm", 0);
+    if (L) {   // landmark table [n][L], farthest-first from vertex 0 (identity numbering)
+        u32* lm = dalloc<u32>((size_t)n * L);
+        std::vector<u32> d(n), mind(n, 0xFFFFFFFFu);
+        or_dijkstra(n, row.data(), col.data(), wt.data(), 0, d.data(), nullptr, nullptr, nullptr);
+        auto far = [&](const std::vector<u32>& a) { u32 b = 0, bd = 0; for (u32 v = 0; v < n; ++v) if (a[v] != 0xFFFFFFFFu && a[v] > bd) { bd = a[v]; b = v; } return b; };
+        u32 cur = far(d);
+        for (u32 j = 0; j < L; ++j) {
+            or_dijkstra(n, row.data(), col.data(), wt.data(), cur, d.data(), nullptr, nullptr, nullptr);
+            for (u32 v = 0; v < n; ++v) { lm[(size_t)v * L + j] = d[v]; mind[v] = std::min(mind[v], d[v]); }
+            cur = far(mind);
+        }
+        r.lm = lm; r.lmCount = L; r.lmSym = gi_("sym", 1);
+        if (astar) r.hkind = HK_ALT;
+    }
+    r.waveCutoff = 64; r.blockCutoff = bs;
+    r.tileItems = tileItems; r.numTiles = (u32)((Kn + tileItems - 1) / tileItems); r.total = (u32)Kn;
+    r.bar = dalloc<unsigned long long>(256, true);
+    r.spinTicks = 200000000ull;
+    std::vector<u32> srcs(K);
+    for (u32 k = 0; k < K; ++k) srcs[k] = (u32)((k * 2654435761ull + 7) % n);
+    if (src0 >= 0) srcs[0] = (u32)src0;
+    std::vector<u32> seeds(K);
+    for (u32 k = 0; k < K; ++k) { seeds[k] = k * n + srcs[k]; target[k] = (srcs[k] * 7 + 13) % n; }
+    u32 startPhase = PH_LIGHT, mode = 0, useBound = 1;
+    if (algo == "frontier") { startPhase = PH_FRONTIER; useBound = 0; }
+    if (algo == "naive") { startPhase = PH_NAIVE; mode = 1; useBound = 0; }
+    if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }
+    const u32 maxSteps = 100000;
+    // initKernel + seedKernel
+    emuLaunch(4, 64, false, [&](u32) {
+        initKernel(r.state, Kn, r.nearMask[0], r.nearMask[1], r.farMask, r.rMask, maskWords(Kn), r.flags, Kn,
+                   algo == "naive" ? r.act[0] : nullptr, r.act[1], n);
+    });
+    emuLaunch((K + 127) / 128, 128, false, [&](u32) { seedKernel(r, seeds.data(), K, mode, startPhase, maxSteps, useBound); });
+    // steps: persistent (one launch, concurrent workgroups) or host-driven (concurrent if grid <= 4)
+    u32 launches = 0;
+    if (persistent) {
+        if (grid > 4) { std::printf("persistent grid <= 4\n"); return 2; }
+        emuLaunch(grid, bs, true, [&](u32 b) { stepB(b, r, 1u, am, map, astar); });
+        launches = 1;
+    } else {
+        while (r.ctl->phase != PH_DONE && launches < 200000) {
+            emuLaunch(grid, bs, grid <= 4, [&](u32 b) { stepB(grid <= 4 ? b : 0, r, 0u, am, map, astar); });
+            ++launches;
+        }
+    }
+    std::printf("phase %u steps %u error %u launches %u\n", r.ctl->phase, r.ctl->steps, r.ctl->error, launches);
+    // check
+    u64 bad = 0;
+    std::vector<u32> d(n), pc(n);
+    for (u32 k = 0; k < K; ++k) {
+        or_dijkstra(n, row.data(), col.data(), wt.data(), srcs[k], d.data(), nullptr, pc.data(), nullptr);
+        for (u32 v = 0; v < n; ++v) {
+            const u64 s = r.state[(u64)k * n + v];
+            if (astar) {
+                if (v == target[k] && (u32)(s >> 32) != d[v]) ++bad;
+                if (v == target[k] && d[v] != 0xFFFFFFFFu) {   // canonical path: every parent on it
+                    u32 x = v, guard = 0;
+                    while (x != srcs[k] && guard++ <= n) {
+                        const u32 p = (u32)r.state[(u64)k * n + x];
+                        if (p != pc[x]) { ++bad; break; }
+                        x = p;
+                    }
+                }
+            } else if ((u32)(s >> 32) != d[v] || (u32)s != pc[v]) {
+                if (bad < 5) std::printf("slot %u v %u: got (%u,%u) want (%u,%u)\n", k, v, (u32)(s >> 32), (u32)s, d[v], pc[v]);
+                ++bad;
+            }
+        }
+    }
+    std::printf("%s: n %u m %u algo %s map %d am %d bs %u grid %u K %u delta %u pers %d astar %u -> %llu mismatches\n",
+                bad ? "FAIL" : "ok", n, m, algo.c_str(), map, am, bs, grid, K, delta, persistent, astar,
+                (unsigned long long)bad);
+    return bad ? 1 : 0;
+}

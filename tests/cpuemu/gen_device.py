@@ -1,0 +1,33 @@
+"""pf_device.hpp -> a header the CPU emulation compiles (test infrastructure).
+
+    python gen_device.py NAMESPACE OUT.hpp
+
+Removes the HIP include, renames namespace pf, gives the dynamic LDS array a static
+size, and replaces the two constructs that have no CPU meaning: the wave-aggregated
+queue increment (one atomic per lane instead: same queue contents, another order)
+and the register-dependency zero (a fence and a zero).
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "..", "..", "pathfinding_gpu_amd", "csrc", "pf_device.hpp")
+
+
+def generate(ns: str) -> str:
+    src = open(SRC).read()
+    src = src.replace("#pragma once", "").replace("#include <hip/hip_runtime.h>", "")
+    src = src.replace("namespace pf {", "namespace %s {" % ns)
+    src = src.replace("extern __shared__ GatherSlot gather[];", "static GatherSlot gather[1024];")
+    a = src.index("__device__ __forceinline__ u32 waveAggInc(u32* ctr) {")
+    b = src.index("}\n", a) + 2
+    src = src[:a] + "__device__ __forceinline__ u32 waveAggInc(u32* ctr) { return atomicAdd(ctr, 1u); }\n" + src[b:]
+    a = src.index("__device__ __forceinline__ u32 zeroAfter(u32 v) {")
+    b = src.index("}\n", a) + 2
+    src = src[:a] + ("__device__ __forceinline__ u32 zeroAfter(u32 v) "
+                     "{ std::atomic_thread_fence(std::memory_order_seq_cst); return v & 0u; }\n") + src[b:]
+    return src
+
+
+if __name__ == "__main__":
+    open(sys.argv[2], "w").write(generate(sys.argv[1]))

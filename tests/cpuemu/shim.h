@@ -1,0 +1,236 @@
+// CPU emulation of the HIP device features used by pathfinding_gpu_amd/csrc/pf_device.hpp
+// (test infrastructure).  Every lane is a fiber on one OS thread: __syncthreads and the wave
+// operations (__shfl*, __ballot) are barriers of the workgroup's or the wave's fibers, and a
+// lane may also be preempted at any atomic or device-scope access (emuYieldEvery), so
+// workgroups and lanes interleave at memory-operation granularity.  Memory is sequentially
+// consistent: this checks the kernels' logic and bounds (under AddressSanitizer), not the
+// GPU's memory model or its code generation.  A wave operation that not all 64 lanes reach
+// never completes; the watchdog reports it.
+#pragma once
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+struct dim3 { uint32_t x = 1, y = 1, z = 1; };
+struct uint2 { uint32_t x, y; };
+struct uint4 { uint32_t x, y, z, w; };
+struct int2 { int x, y; };
+struct alignas(16) double2 { double x, y; };
+inline uint2 make_uint2(uint32_t a, uint32_t b) { return {a, b}; }
+inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return {a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return {a, b}; }
+inline double2 make_double2(double a, double b) { return {a, b}; }
+
+// ---- fibers: every lane is a ucontext fiber on one OS thread; barriers yield ----
+#include <ucontext.h>
+#include <setjmp.h>
+struct EmuBar {
+    uint32_t count = 0, arrived = 0;
+    uint64_t gen = 0;
+};
+struct EmuWave {
+    EmuBar bar;
+    uint64_t slot[64];
+};
+struct EmuBlock {
+    EmuBar bar;
+    std::vector<EmuWave*> waves;
+    int orv[1024];
+};
+struct EmuCtx {
+    uint32_t tid, bid, bdim, gdim;
+    EmuBlock* blk;
+};
+inline EmuCtx ectx;
+struct EmuFiber {
+    ucontext_t uc;
+    jmp_buf jb;
+    bool started = false;
+    char* stack;
+    bool done = false;
+    EmuBar* waitBar = nullptr;
+    uint64_t waitGen = 0;
+    EmuCtx cx;
+    std::function<void()> fn;
+};
+inline ucontext_t emuMain;
+inline jmp_buf emuMainJb;
+inline EmuFiber* emuCur = nullptr;
+inline void emuYield() {
+    if (!_setjmp(emuCur->jb)) _longjmp(emuMainJb, 1);
+}
+inline uint64_t emuRng = 0x9E3779B97F4A7C15ull;
+inline uint32_t emuYieldEvery = 0;   // 0: never preempt; else yield on ~1/emuYieldEvery memory operations
+inline void emuMaybeYield() {
+    if (!emuYieldEvery || !emuCur) return;
+    emuRng ^= emuRng << 13; emuRng ^= emuRng >> 7; emuRng ^= emuRng << 17;
+    if (emuRng % emuYieldEvery == 0) emuYield();
+}
+inline void emuArrive(EmuBar& b) {
+    const uint64_t g = b.gen;
+    if (++b.arrived == b.count) { b.arrived = 0; ++b.gen; return; }
+    emuCur->waitBar = &b;
+    emuCur->waitGen = g;
+    while (b.gen == g) emuYield();
+    emuCur->waitBar = nullptr;
+}
+struct EmuX { uint32_t x; };
+#define threadIdx (EmuX{ectx.tid})
+#define blockIdx (EmuX{ectx.bid})
+#define blockDim (EmuX{ectx.bdim})
+#define gridDim (EmuX{ectx.gdim})
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __align__(x) alignas(x)
+#define __shared__ static
+#define __HIP_MEMORY_SCOPE_AGENT 0
+
+inline EmuWave& emuWave() { return *ectx.blk->waves[ectx.tid >> 6]; }
+inline void __syncthreads() { emuArrive(ectx.blk->bar); }
+inline int __syncthreads_or(int p) {
+    ectx.blk->orv[ectx.tid] = p ? 1 : 0;
+    __syncthreads();
+    int r = 0;
+    for (uint32_t i = 0; i < ectx.bdim; ++i) r |= ectx.blk->orv[i];
+    __syncthreads();
+    return r;
+}
+inline uint32_t __lane_id() { return ectx.tid & 63u; }
+template <class T>
+T emuShflFrom(T v, int src) {
+    EmuWave& w = emuWave();
+    uint64_t b = 0;
+    static_assert(sizeof(T) <= 8, "shfl");
+    std::memcpy(&b, &v, sizeof(T));
+    w.slot[ectx.tid & 63u] = b;
+    emuArrive(w.bar);
+    const uint64_t r = w.slot[src & 63];
+    emuArrive(w.bar);
+    T out;
+    std::memcpy(&out, &r, sizeof(T));
+    return out;
+}
+template <class T> T __shfl(T v, int src, int = 64) { return emuShflFrom(v, src); }
+template <class T> T __shfl_up(T v, unsigned off, int = 64) {
+    const int lane = (int)(ectx.tid & 63u);
+    return emuShflFrom(v, lane >= (int)off ? lane - (int)off : lane);
+}
+template <class T> T __shfl_down(T v, unsigned off, int = 64) {
+    const int lane = (int)(ectx.tid & 63u);
+    return emuShflFrom(v, lane + (int)off < 64 ? lane + (int)off : lane);
+}
+inline unsigned long long __ballot(int p) {
+    EmuWave& w = emuWave();
+    w.slot[ectx.tid & 63u] = p ? 1 : 0;
+    emuArrive(w.bar);
+    unsigned long long m = 0;
+    for (int i = 0; i < 64; ++i) m |= (w.slot[i] ? 1ull : 0ull) << i;
+    emuArrive(w.bar);
+    return m;
+}
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline double __dsqrt_rn(double x) { return std::sqrt(x); }
+inline unsigned long long wall_clock64() {
+    return (unsigned long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch()).count() / 10ull;
+}
+#define __builtin_amdgcn_s_waitcnt(x) std::atomic_thread_fence(std::memory_order_seq_cst)
+#define __builtin_amdgcn_s_sleep(x) emuYield()
+
+template <class T> T __hip_atomic_load(T* p, int, int) { emuMaybeYield(); emuMaybeYield(); return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+template <class T, class V> void __hip_atomic_store(T* p, V v, int, int) { emuMaybeYield(); __atomic_store_n(p, (T)v, __ATOMIC_SEQ_CST); }
+inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { emuMaybeYield(); return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { emuMaybeYield(); return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline uint32_t atomicOr(uint32_t* p, uint32_t v) { emuMaybeYield(); return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline uint32_t atomicAnd(uint32_t* p, uint32_t v) { emuMaybeYield(); return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicCAS(unsigned long long* p, unsigned long long c, unsigned long long v) {
+    emuMaybeYield();
+    __atomic_compare_exchange_n(p, &c, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return c;
+}
+inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v) {
+    emuMaybeYield();
+    unsigned long long cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
+inline uint32_t atomicMin(uint32_t* p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
+
+// Launch: every lane a fiber; workgroups one after another (concurrent=false) or all at once.
+inline void emuFiberEntry(unsigned lo, unsigned hi) {
+    EmuFiber* f = (EmuFiber*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    f->fn();
+    f->done = true;
+    _longjmp(emuMainJb, 1);
+}
+inline void emuRun(uint32_t grid, uint32_t block, uint32_t b0, uint32_t b1, const std::function<void(uint32_t)>& body) {
+    const size_t STACK = 256 * 1024;
+    std::vector<EmuBlock> blocks(b1 - b0);
+    std::vector<EmuFiber*> fibers;
+    for (uint32_t b = b0; b < b1; ++b) {
+        EmuBlock& B = blocks[b - b0];
+        B.bar.count = block;
+        for (uint32_t w = 0; w < block / 64; ++w) { B.waves.push_back(new EmuWave()); B.waves.back()->bar.count = 64; }
+        for (uint32_t t = 0; t < block; ++t) {
+            EmuFiber* f = new EmuFiber();
+            f->stack = (char*)std::malloc(STACK);
+            f->cx = EmuCtx{t, b, block, grid, &B};
+            f->fn = [&body, b] { body(b); };
+            getcontext(&f->uc);
+            f->uc.uc_stack.ss_sp = f->stack;
+            f->uc.uc_stack.ss_size = STACK;
+            f->uc.uc_link = nullptr;
+            makecontext(&f->uc, (void (*)())emuFiberEntry, 2, (unsigned)(uintptr_t)f, (unsigned)((uintptr_t)f >> 32));
+            fibers.push_back(f);
+        }
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t live = fibers.size();
+    while (live) {
+        for (EmuFiber* f : fibers) {
+            if (f->done) continue;
+            if (f->waitBar && f->waitBar->gen == f->waitGen) continue;
+            emuCur = f;
+            ectx = f->cx;
+            if (!_setjmp(emuMainJb)) {
+                if (!f->started) { f->started = true; swapcontext(&emuMain, &f->uc); }
+                else _longjmp(f->jb, 1);
+            }
+            if (f->done) --live;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
+            std::printf("EMU DEADLOCK/TIMEOUT: %zu lanes still live\n", live);
+            std::exit(3);
+        }
+    }
+    for (EmuFiber* f : fibers) { std::free(f->stack); delete f; }
+    for (auto& B : blocks) for (auto* w : B.waves) delete w;
+}
+inline void emuLaunch(uint32_t grid, uint32_t block, bool concurrent, const std::function<void(uint32_t)>& body) {
+    if (concurrent) { emuRun(grid, block, 0, grid, body); return; }
+    // one workgroup's lanes run the workgroups one after another
+    emuRun(grid, block, 0, 1, [&](uint32_t) {
+        for (uint32_t b = 0; b < grid; ++b) {
+            emuCur->cx.bid = b;
+            ectx.bid = b;
+            body(b);
+            __syncthreads();
+        }
+    });
+}

@@ -1,0 +1,84 @@
+"""The device code on the CPU (no GPU): the step kernel of pf_device.hpp compiled with g++
+against a fiber-based emulation of workgroups and waves (tests/cpuemu), under
+AddressSanitizer and UBSan, with device buffers filled with garbage as hipMalloc leaves
+them and random preemption of lanes at memory operations.  Each case checks dist and
+parent (A*: target distance and canonical path) against the oracle.
+
+What this covers: the kernels' logic, every index they form and every bound they
+assume, and interleavings of workgroups at memory-operation granularity under a
+sequentially consistent memory.  What it does not: the GPU's memory model and code
+generation (those are the GPU tests' job).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "cpuemu")
+BUILD = os.path.join(EMU, "_build")
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def emu(orc):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    os.makedirs(BUILD, exist_ok=True)
+    sys.path.insert(0, EMU)
+    try:
+        import gen_device
+    finally:
+        sys.path.pop(0)
+    for i in range(4):
+        with open(os.path.join(BUILD, f"dev_emu_{i}.hpp"), "w") as f:
+            f.write(gen_device.generate(f"pf{i}"))
+    exe = os.path.join(BUILD, "emu_asan")
+    lib_graph = os.path.join(ROOT, "pathfinding_gpu_amd")
+    lib_oracle = os.path.join(ROOT, "oracle")
+    if not os.path.exists(os.path.join(lib_graph, "libpfgraph.so")):
+        pytest.skip("libpfgraph.so not built")
+    cmd = ["g++", "-std=c++20", "-O1", "-g0", "-w", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0",
+           "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+           "-I", EMU, "-I", BUILD, "-I", os.path.join(ROOT, "include"), "-I", lib_oracle,
+           "-o", exe, os.path.join(EMU, "emu.cpp"),
+           "-L", lib_graph, "-L", lib_oracle, "-lpfgraph", "-loracle",
+           f"-Wl,-rpath,{lib_graph}", f"-Wl,-rpath,{lib_oracle}", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+# (graph, size, algo, extra options): every algorithm, mapping, atomic mode, drive, batch,
+# heuristic and compaction the engine has, on small graphs
+CASES = [
+    ("grid", 20, "naive", "map=0"),
+    ("rmat", 9, "naive", "am=1"),
+    ("grid", 20, "frontier", "map=2 grid=3"),
+    ("rmat", 9, "frontier", "map=1 grid=3 pers=1"),
+    ("grid", 20, "scan", "map=2 grid=3"),
+    ("rmat", 9, "scan", "map=2 grid=3 pers=1 yield=5"),
+    ("grid", 20, "delta", "map=0 grid=4 pers=1 K=3 kmax=4"),
+    ("rmat", 9, "delta", "map=2 grid=3"),
+    ("rmat", 9, "delta", "map=2 am=1 grid=4 pers=1 K=2 kmax=4"),
+    ("grid", 20, "delta", "map=1 am=2 grid=2 pers=1"),
+    ("grid", 20, "delta", "map=2 grid=4 pers=1 K=3 kmax=4 yield=3 fill=255"),
+    ("grid", 20, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1"),
+    ("grid", 20, "frontier", "map=2 grid=3 pers=1 K=2 kmax=4 astar=1"),
+    ("grid", 20, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=8"),
+    ("rmat", 9, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=4 sym=0"),
+    ("grid", 20, "delta", "map=0 grid=2 K=2 kmax=4 astar=1 lm=8 yield=4 delta=8000"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[2]}-{c[3].replace(' ', ',')}")
+def test_device_code_on_cpu(emu, case):
+    g, size, algo, extra = case
+    args = [emu, f"g={g}", f"size={size}", f"algo={algo}", "bs=64"] + extra.split()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
+    out = r.stdout + r.stderr
+    assert r.returncode == 0 and "ok:" in out, out[-4000:]
