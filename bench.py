@@ -198,7 +198,7 @@ def main():
     achieved = bytes_launch / (loop_ms * 1e-3) / 1e9
     timed_opts = {"delta": int(learnedK.delta), "mapping": int(learnedK.mapping), "block_size": int(learnedK.block_size),
                   "wave_cutoff": int(learnedK.wave_cutoff), "block_cutoff": int(learnedK.block_cutoff),
-                  "grid_blocks": int(learnedK.grid_blocks)}
+                  "grid_blocks": int(learnedK.grid_blocks), "async_light": int(learnedK.async_light)}
     traffic = measured_traffic(timed_opts) if args.workload == "grid1000" and K == 64 else None
 
     out = {
@@ -233,7 +233,8 @@ def main():
         "host_driven_reference_style_ms": round(ms_host, 3),
         "learned_single_ms": round(ms_l1, 3),
         "learned_single_options": {"delta": int(learned1.delta), "mapping": int(learned1.mapping),
-                                   "block_size": int(learned1.block_size), "grid_blocks": int(learned1.grid_blocks)},
+                                   "block_size": int(learned1.block_size), "grid_blocks": int(learned1.grid_blocks),
+                                   "async_light": int(learned1.async_light)},
         "learned_single_gteps": round(arcs1 / (ms_l1 * 1e-3) / 1e9, 5),
         "learned_single_speedup": round(ms_plain / ms_l1, 3),
         "step_stats": {"steps": r_step.stats.steps, "buckets": r_step.stats.buckets,
@@ -263,11 +264,12 @@ def main():
         # wider buckets pay off; judged once by timing among 1x, 4x and 8x the Davidson default
         alt_ms = {}
         for mult in (1, 4, 8):
-            o_ = opt(auto * mult, pickA["mapping"], heuristic=P.Heuristic.ALT)
-            sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)
-            alt_ms[mult] = sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)["ms"]
-        alt_mult = min(alt_ms, key=alt_ms.get)
-        lopt = opt(auto * alt_mult, pickA["mapping"], heuristic=P.Heuristic.ALT)
+            for asy in (False, True):
+                o_ = opt(auto * mult, pickA["mapping"], heuristic=P.Heuristic.ALT, async_light=asy)
+                sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)
+                alt_ms[(mult, asy)] = sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)["ms"]
+        alt_mult, alt_async = min(alt_ms, key=alt_ms.get)
+        lopt = opt(auto * alt_mult, pickA["mapping"], heuristic=P.Heuristic.ALT, async_light=alt_async)
         ra = sv.astar_batch(pa[:, 0], pa[:, 1], lopt, path_cap=8192)
         re_ = sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192)
         alt_same = bool(np.array_equal(ra["dist"], re_["dist"]) and
@@ -275,7 +277,8 @@ def main():
         l_ts, _ = timed(lambda: sv.astar_batch(pa[:, 0], pa[:, 1], lopt, path_cap=8192), max(2, args.steps // 3))
         l_ms = float(np.mean(l_ts))
         out["astar_alt"] = {"queries": QA, "landmarks": len(lm_ids), "delta": int(auto * alt_mult),
-                            "delta_judged_ms": {f"{m}x": round(v, 2) for m, v in alt_ms.items()},
+                            "async_light": alt_async,
+                            "judged_ms": {f"{m}x{'/async' if a else ''}": round(v, 2) for (m, a), v in alt_ms.items()},
                             "batch_ms": round(l_ms, 3),
                             "queries_per_s": round(QA / (l_ms * 1e-3), 1),
                             "edges_scanned": ra["stats"].edges_scanned,

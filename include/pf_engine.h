@@ -62,7 +62,10 @@ typedef struct {
     uint32_t grid_blocks;  /* workgroups per step launch (0 = automatic) */
     int32_t heuristic;     /* pf_heuristic for A*; default PF_H_SCALED */
     uint32_t max_steps;    /* 0 = automatic cap */
-    uint32_t reserved[3];
+    uint32_t async_light;  /* Delta-stepping: 1 = each bucket's light relaxations as one asynchronous
+                              work-queue step (no grid barrier per light iteration), 0 = one step per
+                              light iteration as the reference; identical results either way */
+    uint32_t reserved[2];
 } pf_options;
 
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */

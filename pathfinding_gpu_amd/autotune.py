@@ -10,7 +10,9 @@ Here they are searched per graph and batch size on the device itself:
   * block_cutoff      -- degree from which a whole workgroup expands a vertex,
   * grid_blocks       -- workgroups of the persistent launch, i.e. how the
                          frontier queue is chunked over the machine,
-  * mapping           -- thread / wave / hybrid.
+  * mapping           -- thread / wave / hybrid,
+  * async_light       -- Delta-stepping's light phase as one asynchronous work-queue
+                         step per bucket, or one step per light iteration.
 
 Search: coordinate descent from the engine defaults, each trial the best of
 `reps` timed runs of the caller's representative workload; results are cached
@@ -35,6 +37,7 @@ SPACE = {
     "block_cutoff_mult": (1, 4, 16),
     "grid_mult": (0.5, 1, 2, 4),
     "mapping": (0, 1, 2),
+    "async_light": (0, 1),
 }
 
 
@@ -45,17 +48,19 @@ class Tuned:
     block_cutoff_mult: int = 1
     grid_mult: float = 2
     mapping: int = 2
+    async_light: int = 0
     ms: float = float("inf")
     trials: int = 0
 
     def options(self, cus: int, **kw) -> GpuOptions:
         return GpuOptions(drive=Drive.Persistent, block_size=self.block_size, wave_cutoff=self.wave_cutoff,
                           block_cutoff=self.block_size * self.block_cutoff_mult,
-                          grid_blocks=max(1, int(self.grid_mult * cus)), mapping=Mapping(self.mapping), **kw)
+                          grid_blocks=max(1, int(self.grid_mult * cus)), mapping=Mapping(self.mapping),
+                          async_light=bool(self.async_light), **kw)
 
 
 def signature(g: CSRGraph, batch: int, delta: int) -> str:
-    return f"{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
+    return f"v2:{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
 
 
 def autotune(g: CSRGraph, batch: int, delta: int, cus: int, time_fn: Callable[[GpuOptions], float],

@@ -84,6 +84,8 @@ struct pf_solver {
     uint4* d_vrec = nullptr;   // {row_offsets[u], light end of row u, original id, 0} [n+1]
     u32* d_inv = nullptr;      // original id -> internal id (null: identity)
     u32* d_lm = nullptr;       // landmark distances [n][lmCount], internal order (ALT heuristic)
+    u32* d_ring = nullptr;     // work queue of the asynchronous light phase
+    u64 ringCap = 0;
     u32 lmCount = 0, lmSym = 0;
     std::vector<u32> perm, inv; // internal -> original, original -> internal (empty: identity)
     std::vector<u32> callerRow; // the caller's row_offsets (test hooks report in the caller's CSR)
@@ -240,6 +242,7 @@ u32 inputSize(const Ctl& c, u32 n, u32 numTiles, u32 bs) {
         case PH_FARMIN: case PH_PART: return divUp(c.cnt[C_FAR0 + c.fcur], bs);
         case PH_SCAN_TILES: case PH_SCAN_SCATTER: return numTiles;
         case PH_NAIVE: return divUp(n, bs);
+        case PH_LIGHT_ASYNC: return 0xFFFFFFFFu;   // every workgroup is a worker
         default: return 1;
     }
 }
@@ -429,6 +432,21 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     u32 startPhase = PH_LIGHT;
     u32 mode = 0;
     bool useBound = true;
+    // asynchronous light phase: a work queue of twice the run's K*n slots (a bucket's pushes,
+    // re-pushes included); a bucket that needs more is rerun below with the synchronous phase
+    const bool async = o.algo == PF_ALGO_DELTA && o.async_light != 0;
+    if (async) {
+        const u64 cap = 2 * Kn;
+        if (cap > s->ringCap) {
+            if (s->d_ring) { hipFree(s->d_ring); s->bytes -= s->ringCap * 4; s->d_ring = nullptr; s->ringCap = 0; }
+            if ((rc = s->dalloc(&s->d_ring, cap))) return rc;
+            s->ringCap = cap;
+        }
+        run.ring = s->d_ring;
+        run.ringCap = cap;
+        run.async = 1;
+        startPhase = PH_LIGHT_ASYNC;
+    }
     if (o.algo == PF_ALGO_NAIVE) { startPhase = PH_NAIVE; mode = 1; useBound = false; }
     else if (o.algo == PF_ALGO_FRONTIER) { startPhase = scanMode ? PH_SCAN_RELAX : PH_FRONTIER; useBound = false; }
     const u64 capSteps = std::min<u64>(0xFFFFFF00ull, 16ull * Kn + 100000ull);
@@ -449,6 +467,11 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
                            s->d_nearMask[1], s->d_farMask, s->d_rMask, words, scanMode ? s->d_flags : nullptr, Kn,
                            o.algo == PF_ALGO_NAIVE ? s->d_act[0] : nullptr, s->d_act[1], s->n);
         PF_HIP(hipGetLastError());
+        if (async) {
+            hipLaunchKernelGGL(fillKernel, dim3(std::min<u32>(divUp(run.ringCap, 256), 8192u)), dim3(256), 0, s->s,
+                               s->d_ring, run.ringCap, RING_EMPTY);
+            PF_HIP(hipGetLastError());
+        }
         const u32 sg = divUp(std::max(spec.nseeds, spec.k), 128);
         hipLaunchKernelGGL(seedKernel, dim3(sg), dim3(128), 0, s->s, run, s->d_seeds, spec.nseeds, mode, startPhase,
                            maxSteps, useBound ? 1u : 0u);
@@ -515,6 +538,11 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         std::snprintf(buf, sizeof(buf), "grid barrier wait exceeded its spin limit (tickets %llu, released %llu, phase %u, steps %u)",
                       b[0], b[BAR_GEN], hc.phase, hc.steps);
         return fail(PF_ERR_HIP, buf);
+    }
+    if (hc.error & E_RING_FULL) {   // a bucket outgrew the work queue: the synchronous phase has no such limit
+        RunSpec again = spec;
+        again.o.async_light = 0;
+        return doRun(s, again, st);
     }
     if (hc.error & E_NOT_CONVERGED || hc.phase != PH_DONE) return fail(PF_ERR_NOT_CONVERGED, "relaxation did not converge");
     s->lastK = spec.k;
@@ -911,7 +939,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_ring, s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,

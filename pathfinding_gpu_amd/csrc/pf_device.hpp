@@ -47,12 +47,13 @@ enum Phase : u32 {
     PH_SCAN_RELAX = 6,   // frontier pass that only sets flags
     PH_SCAN_TILES = 7,   // per-tile flag sums
     PH_SCAN_SCATTER = 8, // tile-local scan + scatter into the next frontier
-    PH_DONE = 9
+    PH_DONE = 9,
+    PH_LIGHT_ASYNC = 10  // the bucket's light relaxations as one asynchronous work-queue step
 };
 
 enum Counter : u32 { C_NEAR0 = 0, C_NEAR1 = 1, C_FAR0 = 2, C_FAR1 = 3, C_R = 4, C_CHANGED = 5, C_COUNT = 8 };
 
-enum CtlError : u32 { E_NONE = 0, E_NOT_CONVERGED = 1, E_BARRIER = 2 };
+enum CtlError : u32 { E_NONE = 0, E_NOT_CONVERGED = 1, E_BARRIER = 2, E_RING_FULL = 4 };
 
 // Control block (HBM).  Fields are read by every workgroup at the start of a
 // step and written only by the controller (the step's last workgroup).
@@ -112,9 +113,16 @@ struct Run {
     const u32* lm;               // landmark distances [n][lmCount] (HK_ALT), internal vertex order
     u32 lmCount;                 // landmarks per vertex, a multiple of 4, <= 16
     u32 lmSym;                   // graph is symmetric: distances to a landmark equal distances from it
+    u32* ring;                   // asynchronous light phase: work queue [ringCap], RING_EMPTY when free
+    u64 ringCap;
+    u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
+// Work-queue counters of the asynchronous light phase, one 128-byte line each (in the barrier
+// buffer): pushes reserved (tail), claims taken (head), items finished (done).
+constexpr u32 RING_TAIL = 32, RING_HEAD = 48, RING_DONE = 64;
+constexpr u32 RING_EMPTY = 0xFFFFFFFFu;
 
 struct LocalStats {
     u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
@@ -183,6 +191,16 @@ __device__ __forceinline__ u32 waveAggInc(u32* ctr) {
     return base + rank;
 }
 __device__ __forceinline__ void pushQ(u32* data, u32* ctr, u32 item) { stA(&data[waveAggInc(ctr)], item); }
+
+// 64-bit wave-aggregated increment (work-queue counters).
+__device__ __forceinline__ u64 waveAggInc64(unsigned long long* ctr) {
+    const u64 active = __builtin_amdgcn_ballot_w64(true);
+    const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(active >> 32), __builtin_amdgcn_mbcnt_lo((u32)active, 0u));
+    unsigned long long base = 0;
+    if (rank == 0) base = atomicAdd(ctr, (unsigned long long)__popcll(active));
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)base), hi = __builtin_amdgcn_readfirstlane((u32)(base >> 32));
+    return (((u64)hi << 32) | lo) + rank;
+}
 
 // Read-then-atomicOr dedup bit (src/gpu_kernels.cuh:124-129).
 // Mask words change across the steps of one persistent launch, whose vector L1
@@ -314,7 +332,22 @@ struct Outs {
     u32* farCnt;
     u32* farMask;
     u32* flags;   // prefix-scan mode: set flags[gv] instead of pushing
+    u32* ring;    // asynchronous light phase: near pushes go to the work queue
 };
+
+// Push onto the work queue of the asynchronous light phase.  The slot is written only after
+// its index has come back from the tail counter (address dependency), so an item is never
+// visible before its reservation.  A full queue drops the push, counts it finished and flags
+// the run (the host then reruns it with the synchronous light phase).
+__device__ __forceinline__ void pushRing(const Run& r, u32 gv) {
+    const u64 idx = waveAggInc64(&r.bar[RING_TAIL]);
+    if (idx < r.ringCap) {
+        stA(&r.ring[idx], gv);
+    } else {
+        atomicOr(&r.ctl->error, (u32)E_RING_FULL);
+        atomicAdd(&r.bar[RING_DONE], 1ull);
+    }
+}
 
 // Per-vertex context of an expanded frontier element.
 struct Item {
@@ -342,7 +375,10 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
             stA(&o.flags[gv], 1u);
             ls.pushes++;
         } else if (key < it.bnd) {
-            if (testAndSetBit(o.nearMask, gv)) { pushQ(o.nearQ, o.nearCnt, gv); ls.pushes++; }
+            if (testAndSetBit(o.nearMask, gv)) {
+                if (o.ring) pushRing(r, gv); else pushQ(o.nearQ, o.nearCnt, gv);
+                ls.pushes++;
+            }
         } else {
             if (testAndSetBit(o.farMask, gv)) { pushQ(o.farQ, o.farCnt, gv); ls.pushes++; }
         }
@@ -500,6 +536,124 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
     }
 }
 
+// ---------------------------------------------------------------------------
+// Asynchronous light phase: the bucket's light relaxations in one step, as a
+// work queue (no grid barrier between light iterations).  Items are element
+// ids k*n+v in r.ring[0, tail); a lane claims the next index from the head
+// counter and waits until that slot is filled, takes the item (resetting the
+// slot), expands it with its light arcs -- pushing newly improved near
+// vertices onto the queue, far ones onto the far pile, vertices with heavy
+// arcs into R -- and counts it finished.  The phase ends when the finished
+// count equals the reserved count: read in that order, equality means no item
+// is queued or in progress, and none can be pushed any more.  Each vertex is
+// expanded with the distance it has when its mask bit is cleared (as in the
+// synchronous phase), so the result is the same fixpoint: every distance and
+// min-index parent equals the synchronous phase's.
+// ---------------------------------------------------------------------------
+template <bool ASTAR, int AM>
+__device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
+    const u32 n = r.g.n;
+    Ctl* c = r.ctl;
+    unsigned long long* head = &r.bar[RING_HEAD];
+    unsigned long long* tail = &r.bar[RING_TAIL];
+    unsigned long long* done = &r.bar[RING_DONE];
+    const Outs o{nullptr, nullptr, r.nearMask[0], r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring};
+    constexpr u64 NOCLAIM = ~0ull;
+    u64 my = NOCLAIM;
+    const u32 lane = laneId();
+    const unsigned long long t0 = wall_clock64();
+    while (true) {                                        // wave-uniform
+        // claims for the lanes without one: one atomic per wave
+        const u64 need = __ballot(my == NOCLAIM);
+        if (need) {
+            const int first = __ffsll((unsigned long long)need) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == first) base = atomicAdd(head, (unsigned long long)__popcll(need));
+            base = ((u64)__shfl((u32)(base >> 32), first, 64) << 32) | __shfl((u32)base, first, 64);
+            if (my == NOCLAIM) {
+                const u64 below = need & ((1ull << lane) - 1ull);
+                my = base + (u64)__popcll(below);
+            }
+        }
+        // take the claimed item if it has arrived
+        u32 gi = RING_EMPTY;
+        if (my < r.ringCap) {
+            const u32 v = ldA(&r.ring[my]);
+            if (v != RING_EMPTY) {
+                gi = v;
+                stA(&r.ring[my], RING_EMPTY);
+                my = NOCLAIM;
+            }
+        }
+        const u64 got = __ballot(gi != RING_EMPTY);
+        if (got) {
+            Item it;
+            it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
+            it.ub = NOKEY; it.bnd = NOKEY;
+            if (gi != RING_EMPTY) {
+                it.k = slotOf(r, gi);
+                it.u = gi - it.k * n;
+                const uint4 va = r.g.vrec[it.u];
+                const u32 r1 = r.g.vrec[it.u + 1].x;
+                const u32 z = zeroAfter(clearBitRet(r.nearMask[0], gi));
+                it.du = distOf(ldA(&r.state[gi + z]));
+                bool active = it.du != INF;
+                if constexpr (ASTAR) {
+                    const u32 t = r.target[it.k];
+                    it.t = loadTarget(r, t);
+                    it.ub = distOf(r.state[it.k * n + t]);
+                    if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
+                }
+                if (active) {
+                    it.p = va.z;
+                    if (va.y < r1 && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &c->cnt[C_R], gi);
+                    it.beg = va.x;
+                    it.end = va.y;                       // light arcs
+                    it.bnd = ldA(&r.bound[it.k]);
+                }
+            }
+            bool busy = it.end > it.beg;
+            // wave tier for high light degree, one lane per vertex otherwise
+            const bool big = busy && it.end - it.beg >= r.waveCutoff;
+            u64 ballot = __ballot(big);
+            while (ballot) {
+                const int leader = __ffsll((unsigned long long)ballot) - 1;
+                ballot &= ballot - 1;
+                Item wi;
+                wi.u = __shfl(it.u, leader, 64);
+                wi.p = __shfl(it.p, leader, 64);
+                wi.du = __shfl(it.du, leader, 64);
+                wi.k = __shfl(it.k, leader, 64);
+                wi.beg = __shfl(it.beg, leader, 64);
+                wi.end = __shfl(it.end, leader, 64);
+                wi.ub = ((u64)__shfl((u32)(it.ub >> 32), leader, 64) << 32) | __shfl((u32)it.ub, leader, 64);
+                wi.bnd = ((u64)__shfl((u32)(it.bnd >> 32), leader, 64) << 32) | __shfl((u32)it.bnd, leader, 64);
+                if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
+                relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
+            }
+            if (busy && !big) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
+            // this wave's pushes have been performed before its items count as finished (the empty
+            // asm statements keep the compiler from moving memory operations across the wait)
+            asm volatile("" ::: "memory");
+            waitAll();
+            asm volatile("" ::: "memory");
+            if ((int)lane == __ffsll((unsigned long long)got) - 1) atomicAdd(done, (unsigned long long)__popcll(got));
+            continue;
+        }
+        // nothing taken: finished when done == tail (done read first)
+        u32 stop = 0;
+        if (lane == 0) {
+            const unsigned long long d = ldA(done);
+            const u32 z = zeroAfter((u32)d);
+            const unsigned long long t = ldA(tail + z);
+            if (d >= t) stop = 1;
+            else if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); stop = 1; }
+        }
+        if (__shfl(stop, 0, 64)) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // Naive synchronous relaxation, one thread per vertex (src/gpu_kernels.cuh:297-329).
 template <bool ASTAR, int AM>
 __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, LocalStats& ls) {
@@ -595,7 +749,9 @@ __device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ,
             clearBit(r.farMask, gi);                 // stale or pruned
         } else if (key < ldA(&r.newBound[k])) {
             clearBit(r.farMask, gi);                 // leaves the far pile for the new bucket
-            if (testAndSetBit(nearMask, gi)) pushQ(nearQ, nearCnt, gi);
+            if (testAndSetBit(nearMask, gi)) {
+                if (r.async) pushRing(r, gi); else pushQ(nearQ, nearCnt, gi);
+            }
         } else {
             pushQ(farOut, farOutCnt, gi);            // stays (bit unchanged)
         }
@@ -677,7 +833,8 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         const u32 phase = h.phase;
         u32 in = h.in, fcur = h.fcur;
         u32 next = PH_DONE;
-        u32 resetMin = 0, setBounds = 0, scan = 0;
+        u32 resetMin = 0, setBounds = 0, scan = 0, resetRing = 0;
+        const unsigned long long ringTail = r.async ? ldA(&r.bar[RING_TAIL]) : 0ull;
         auto bucketDone = [&]() {
             h.buckets++;
             if (h.cnt[C_FAR0 + fcur] == 0) next = PH_DONE;
@@ -692,19 +849,38 @@ __device__ void controller(const Run& r, u32* waveTotals) {
                 else if (h.cnt[C_R]) next = PH_HEAVY;
                 else bucketDone();
                 break;
+            case PH_LIGHT_ASYNC:
+                h.iters++;
+                resetRing = 1;                   // every queued item is finished
+                if (h.cnt[C_R]) next = PH_HEAVY;
+                else bucketDone();
+                break;
             case PH_HEAVY:
                 h.iters++;
                 h.cnt[C_R] = 0;
-                if (h.cnt[C_NEAR0 + in]) { next = PH_LIGHT; h.cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else bucketDone();
+                if (r.async) {
+                    if (ringTail) next = PH_LIGHT_ASYNC;
+                    else bucketDone();
+                } else if (h.cnt[C_NEAR0 + in]) {
+                    next = PH_LIGHT;
+                    h.cnt[C_NEAR0 + (in ^ 1u)] = 0;
+                } else {
+                    bucketDone();
+                }
                 break;
             case PH_FARMIN: next = PH_PART; setBounds = 1; break;      // decided over all slots below
             case PH_PART:
                 h.cnt[C_FAR0 + fcur] = 0;
                 fcur ^= 1u;
-                if (h.cnt[C_NEAR0 + in]) { next = PH_LIGHT; h.cnt[C_NEAR0 + (in ^ 1u)] = 0; }
-                else if (h.cnt[C_FAR0 + fcur]) { next = PH_FARMIN; resetMin = 1; }
-                else next = PH_DONE;
+                if (r.async ? ringTail != 0ull : h.cnt[C_NEAR0 + in] != 0u) {
+                    next = r.async ? PH_LIGHT_ASYNC : PH_LIGHT;
+                    if (!r.async) h.cnt[C_NEAR0 + (in ^ 1u)] = 0;
+                } else if (h.cnt[C_FAR0 + fcur]) {
+                    next = PH_FARMIN;
+                    resetMin = 1;
+                } else {
+                    next = PH_DONE;
+                }
                 break;
             case PH_FRONTIER:
                 if (h.cnt[C_NEAR0 + in]) h.iters++;
@@ -724,6 +900,12 @@ __device__ void controller(const Run& r, u32* waveTotals) {
             case PH_SCAN_TILES: next = PH_SCAN_SCATTER; scan = 1; break;
             case PH_SCAN_SCATTER: break;        // total known after the tile scan below
             default: next = PH_DONE;
+        }
+        if (h.error & E_RING_FULL) next = PH_DONE;   // the host reruns with the synchronous light phase
+        if (resetRing) {
+            stA(&r.bar[RING_TAIL], 0ull);
+            stA(&r.bar[RING_HEAD], 0ull);
+            stA(&r.bar[RING_DONE], 0ull);
         }
         h.in = in;
         h.fcur = fcur;
@@ -860,7 +1042,7 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             }
             case PH_HEAVY: {
                 const Outs o{r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
-                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr};
+                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr};
                 expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, &s_owner, &s_item,
                                                gather);
                 break;
@@ -878,6 +1060,7 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
                                                &s_item, gather);
                 break;
             }
+            case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls); break;
             case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
             case PH_SCAN_SCATTER: scanScatter(r, r.nearQ[in ^ 1u], waveTotals); break;
             case PH_FARMIN: farMin<ASTAR>(r, r.farQ[fcur], size); break;
@@ -960,7 +1143,8 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         const u32 gi = seeds[i];
         r.state[gi] = pack(0u, NOPAR);
         if (mode == 0) {
-            r.nearQ[0][i] = gi;
+            if (r.async) r.ring[i] = gi;
+            else r.nearQ[0][i] = gi;
             atomicOr(r.nearMask[0] + (gi >> 5), 1u << (gi & 31u));
         } else {
             r.act[0][gi] = 1;
@@ -991,6 +1175,9 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         c->arrive = 0; c->gen = 0;
         r.bar[0] = 0ull;
         r.bar[BAR_GEN] = 0ull;
+        r.bar[RING_TAIL] = mode == 0 && r.async ? nseeds : 0ull;
+        r.bar[RING_HEAD] = 0ull;
+        r.bar[RING_DONE] = 0ull;
         c->steps = 0; c->iters = 0; c->buckets = 0; c->error = 0;
         c->scanTotal = 0; c->epochEnd = 0; c->pause = 0; c->maxSteps = maxSteps;
         r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
@@ -1123,6 +1310,11 @@ __global__ void lightEndKernel(const u32* row, const uint2* adj, u32 n, u32 delt
         }
         vrec[u].y = lo;
     }
+}
+
+// Fill (grid-stride): the asynchronous phase's work queue starts all RING_EMPTY.
+__global__ void fillKernel(u32* p, u64 count, u32 value) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (u64)gridDim.x * blockDim.x) p[i] = value;
 }
 
 // One landmark's distances into column j of the [n][count] landmark table.

@@ -72,6 +72,7 @@ class GpuOptions:
     block_cutoff: int = 0
     grid_blocks: int = 0
     max_steps: int = 0
+    async_light: bool = False   # Delta: each bucket's light phase as one asynchronous work-queue step
 
     def to_c(self) -> N.PfOptions:
         o = N.PfOptions()
@@ -80,6 +81,7 @@ class GpuOptions:
         o.delta, o.block_size, o.sync_every, o.drive = self.delta, self.block_size, self.sync_every, int(self.drive)
         o.wave_cutoff, o.block_cutoff, o.grid_blocks = self.wave_cutoff, self.block_cutoff, self.grid_blocks
         o.heuristic, o.max_steps = int(self.heuristic), self.max_steps
+        o.async_light = 1 if self.async_light else 0
         return o
 
 

@@ -163,6 +163,13 @@ int main(int argc, char** argv) {
     u32 startPhase = PH_LIGHT, mode = 0, useBound = 1;
     if (algo == "frontier") { startPhase = PH_FRONTIER; useBound = 0; }
     if (algo == "naive") { startPhase = PH_NAIVE; mode = 1; useBound = 0; }
+    if (algo == "delta" && gi_("async", 0)) {   // asynchronous light phase: work queue of 2*K*n slots
+        r.ringCap = gi_("ringcap", 0) ? (u64)gi_("ringcap", 0) : 2 * Kn;
+        r.ring = dalloc<u32>(r.ringCap);
+        for (u64 i = 0; i < r.ringCap; ++i) r.ring[i] = RING_EMPTY;
+        r.async = 1;
+        startPhase = PH_LIGHT_ASYNC;
+    }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }
     const u32 maxSteps = 100000;
     // initKernel + seedKernel

@@ -3,9 +3,9 @@
     python gen_device.py NAMESPACE OUT.hpp
 
 Removes the HIP include, renames namespace pf, gives the dynamic LDS array a static
-size, and replaces the two constructs that have no CPU meaning: the wave-aggregated
-queue increment (one atomic per lane instead: same queue contents, another order)
-and the register-dependency zero (a fence and a zero).
+size, and replaces the constructs that have no CPU meaning: the wave-aggregated queue
+increments (one atomic per lane instead: same queue contents, another order) and
+the register-dependency zero (a fence and a zero).
 """
 import os
 import sys
@@ -22,6 +22,10 @@ def generate(ns: str) -> str:
     a = src.index("__device__ __forceinline__ u32 waveAggInc(u32* ctr) {")
     b = src.index("}\n", a) + 2
     src = src[:a] + "__device__ __forceinline__ u32 waveAggInc(u32* ctr) { return atomicAdd(ctr, 1u); }\n" + src[b:]
+    a = src.index("__device__ __forceinline__ u64 waveAggInc64(unsigned long long* ctr) {")
+    b = src.index("}\n", a) + 2
+    src = src[:a] + ("__device__ __forceinline__ u64 waveAggInc64(unsigned long long* ctr) "
+                     "{ return atomicAdd(ctr, 1ull); }\n") + src[b:]
     a = src.index("__device__ __forceinline__ u32 zeroAfter(u32 v) {")
     b = src.index("}\n", a) + 2
     src = src[:a] + ("__device__ __forceinline__ u32 zeroAfter(u32 v) "

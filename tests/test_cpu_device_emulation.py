@@ -70,6 +70,14 @@ CASES = [
     ("grid", 20, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=8"),
     ("rmat", 9, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=4 sym=0"),
     ("grid", 20, "delta", "map=0 grid=2 K=2 kmax=4 astar=1 lm=8 yield=4 delta=8000"),
+    # asynchronous light phase (work queue per bucket)
+    ("grid", 20, "delta", "async=1 grid=3"),
+    ("grid", 20, "delta", "async=1 map=0 grid=4 pers=1 K=3 kmax=4 yield=3"),
+    ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 yield=5 fill=0"),
+    ("rmat", 9, "delta", "async=1 am=1 grid=3 pers=1 delta=1"),
+    ("grid", 20, "delta", "async=1 am=2 grid=4 pers=1 delta=100000"),
+    ("grid", 20, "delta", "async=1 grid=4 pers=1 K=3 kmax=4 astar=1 lm=8 yield=2"),
+    ("grid", 20, "delta", "async=1 grid=3 pers=1 K=3 kmax=4 astar=1 delta=300"),
 ]
 
 

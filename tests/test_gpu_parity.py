@@ -311,3 +311,55 @@ def test_persistent_full_size_grid(orc, grid1000):
             o = P.GpuOptions(drive=DR.Persistent, delta=auto * mult, mapping=MP.Hybrid)
             r = sv.multi_source(srcs, o)
             assert np.array_equal(r.dist, want.reshape(16, g.n)), f"grid1000 batch delta {auto * mult}"
+
+
+# ---------------------------------------------------------------- asynchronous light phase
+@pytest.mark.parametrize("drive", [DR.Host, DR.Graph, DR.Persistent], ids=lambda a: a.name)
+@pytest.mark.parametrize("mapping", MAPS, ids=lambda a: a.name)
+def test_async_light_phase(orc, grid100, sgrid100, rmat12, srmat12, drive, mapping):
+    """Each bucket's light relaxations as one work-queue step: same dist and parents."""
+    for g, sv, s in ((grid100, sgrid100, 4321), (rmat12, srmat12, rmat12.max_degree_vertex())):
+        for atomic in ATOMS:
+            o = P.GpuOptions(algo=GA.DeltaStep, mapping=mapping, atomic=atomic, drive=drive, async_light=True)
+            r = sv.sssp(s, o)
+            assert_sssp_exact(orc, g, s, r, f"async {drive.name}/{mapping.name}/{atomic.name}")
+            sync = sv.sssp(s, P.GpuOptions(algo=GA.DeltaStep, mapping=mapping, atomic=atomic, drive=drive))
+            if drive == DR.Persistent and g is grid100:
+                assert r.stats.steps < sync.stats.steps
+
+
+@pytest.mark.parametrize("delta", [1, 37, 1000, 1 << 30])
+def test_async_light_delta_values_and_batches(orc, grid100, sgrid100, delta):
+    o = P.GpuOptions(algo=GA.DeltaStep, delta=delta, drive=DR.Persistent, async_light=True)
+    r = sgrid100.sssp(17, o)
+    assert_sssp_exact(orc, grid100, 17, r, f"async delta {delta}")
+    srcs = [0, 99, 5000, 9999, 1234, 42, 7777, 3]
+    ms = sgrid100.multi_source(srcs, o, parents=True)
+    for k, s in enumerate(srcs):
+        d, pc = oracle_sssp(orc, grid100, s)
+        assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), (delta, s)
+
+
+def test_async_light_astar(orc, grid100, sgrid100):
+    rng = np.random.default_rng(11)
+    srcs = rng.integers(0, grid100.n, 8)
+    dsts = rng.integers(0, grid100.n, 8)
+    for drive in (DR.Host, DR.Persistent):
+        res = sgrid100.astar_batch(srcs, dsts, P.GpuOptions(heuristic=HK.FloorEuclid, drive=drive, async_light=True))
+        for i in range(8):
+            d, path, _, _ = oracle_astar(orc, grid100, srcs[i], dsts[i], HK.FloorEuclid)
+            assert res["dist"][i] == d and np.array_equal(res["paths"][i], path)
+
+
+def test_async_light_full_size_grid(orc, grid1000):
+    g = grid1000
+    with P.GpuSolver(g, max_batch=16) as sv:
+        auto = sv.auto_delta()
+        for mult in (1, 32):
+            for mp in (MP.Thread, MP.Hybrid):
+                r = sv.sssp(0, P.GpuOptions(drive=DR.Persistent, delta=auto * mult, mapping=mp, async_light=True))
+                assert_sssp_exact(orc, g, 0, r, f"async grid1000 delta {auto * mult} {mp.name}")
+        srcs = np.random.default_rng(6).choice(g.n, 16, replace=False)
+        want = orc.multi_source(g.row_offsets, g.col_indices, g.weights, srcs, 0)[0]
+        r = sv.multi_source(srcs, P.GpuOptions(drive=DR.Persistent, delta=auto, async_light=True))
+        assert np.array_equal(r.dist, want.reshape(16, g.n))
