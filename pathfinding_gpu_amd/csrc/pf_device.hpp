@@ -536,6 +536,74 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
     }
 }
 
+// One lane's arcs [beg, end) in groups of four: the group's arc loads, its
+// read-before-atomic loads and its atomicMins are each issued together before
+// any result is used, so a group costs three dependent round trips instead of
+// twelve; the queue pushes of the updated arcs follow as in relaxEdge.
+template <bool ASTAR, int AM>
+__device__ __forceinline__ void relaxLane4(const Run& r, const Item& it, const Outs& o, LocalStats& ls) {
+    if constexpr (AM == AM_CAS_RBA) {
+        relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
+    } else {
+        const u32 n = r.g.n;
+        for (u32 e0 = it.beg; e0 < it.end; e0 += 4) {
+            uint2 a[4];
+            bool go[4];
+            u64 key[4], nv[4];
+            u32 gv[4], nd[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = e0 + q < it.end ? r.g.adj[e0 + q] : make_uint2(0u, 0u);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u64 d = (u64)it.du + a[q].y;
+                const bool ok = e0 + q < it.end;
+                ls.edges += ok ? 1u : 0u;
+                go[q] = ok && d < INF;
+                key[q] = d;
+                if constexpr (ASTAR) {
+                    if (go[q]) {
+                        key[q] += heur(r, it.t, a[q].x);
+                        go[q] = key[q] <= it.ub;          // strict: ties with dist[target] stay explored
+                    }
+                }
+                nd[q] = (u32)d;
+                gv[q] = it.k * n + a[q].x;
+                nv[q] = pack(nd[q], it.p);
+            }
+            if constexpr (AM == AM_MIN_RBA) {
+                u64 cur[4];   // a stale value is >= the current one: the check only skips useless atomics
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cur[q] = go[q] ? r.state[gv[q]] : 0ull;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) go[q] = go[q] && cur[q] > nv[q];
+            }
+            u64 old[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                old[q] = 0ull;
+                if (go[q]) {
+                    ls.atomics++;
+                    old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (!(go[q] && distOf(old[q]) > nd[q])) continue;
+                ls.updates++;
+                if (key[q] < it.bnd) {
+                    if (testAndSetBit(o.nearMask, gv[q])) {
+                        if (o.ring) pushRing(r, gv[q]); else pushQ(o.nearQ, o.nearCnt, gv[q]);
+                        ls.pushes++;
+                    }
+                } else if (testAndSetBit(o.farMask, gv[q])) {
+                    pushQ(o.farQ, o.farCnt, gv[q]);
+                    ls.pushes++;
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Asynchronous light phase: the bucket's light relaxations in one step, as a
 // work queue (no grid barrier between light iterations).  Items are element
@@ -631,9 +699,11 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
                 if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
                 relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
             }
-            if (busy && !big) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
-            // this wave's pushes have been performed before its items count as finished (the empty
-            // asm statements keep the compiler from moving memory operations across the wait)
+            if (busy && !big) relaxLane4<ASTAR, AM>(r, it, o, ls);
+            // every lane of the wave is past its relaxations (the wave reconverges here), and the
+            // wave's pushes have been performed, before its items count as finished (the empty asm
+            // statements keep the compiler from moving memory operations across the wait)
+            __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
             waitAll();
             asm volatile("" ::: "memory");

@@ -154,7 +154,7 @@ int main(int argc, char** argv) {
     r.waveCutoff = 64; r.blockCutoff = bs;
     r.tileItems = tileItems; r.numTiles = (u32)((Kn + tileItems - 1) / tileItems); r.total = (u32)Kn;
     r.bar = dalloc<unsigned long long>(256, true);
-    r.spinTicks = 200000000ull;
+    r.spinTicks = 60000000000ull;   // 600 s: emulation is slow; its own watchdog catches a stall
     std::vector<u32> srcs(K);
     for (u32 k = 0; k < K; ++k) srcs[k] = (u32)((k * 2654435761ull + 7) % n);
     if (src0 >= 0) srcs[0] = (u32)src0;
@@ -178,6 +178,16 @@ int main(int argc, char** argv) {
                    algo == "naive" ? r.act[0] : nullptr, r.act[1], n);
     });
     emuLaunch((K + 127) / 128, 128, false, [&](u32) { seedKernel(r, seeds.data(), K, mode, startPhase, maxSteps, useBound); });
+    emuOnTimeout = [&] {
+        std::printf("ring tail %llu head %llu done %llu; ctl phase %u error %u cnt R %u far %u %u\n",
+                    (unsigned long long)r.bar[32], (unsigned long long)r.bar[48], (unsigned long long)r.bar[64],
+                    r.ctl->phase, r.ctl->error, r.ctl->cnt[4], r.ctl->cnt[2], r.ctl->cnt[3]);
+        if (r.ring) {
+            int shown = 0;
+            for (u64 i = 0; i < r.ringCap && shown < 40; ++i)
+                if (r.ring[i] != RING_EMPTY) { std::printf("  ring[%llu] = %u\n", (unsigned long long)i, r.ring[i]); ++shown; }
+        }
+    };
     // steps: persistent (one launch, concurrent workgroups) or host-driven (concurrent if grid <= 4)
     u32 launches = 0;
     if (persistent) {
@@ -211,6 +221,23 @@ int main(int argc, char** argv) {
             } else if ((u32)(s >> 32) != d[v] || (u32)s != pc[v]) {
                 if (bad < 5) std::printf("slot %u v %u: got (%u,%u) want (%u,%u)\n", k, v, (u32)(s >> 32), (u32)s, d[v], pc[v]);
                 ++bad;
+            }
+        }
+    }
+    if (bad && gi_("diag", 0)) {   // violated relaxations: arcs u->v with d(u) + w < d(v) in slot 0
+        int shown = 0;
+        for (u32 u = 0; u < n && shown < 5; ++u) {
+            const u64 su = r.state[u];
+            if ((u32)(su >> 32) == 0xFFFFFFFFu) continue;
+            for (u32 e = r.g.row[u]; e < r.g.row[u + 1]; ++e) {
+                const uint2 a2 = r.g.adj[e];
+                const u64 nd2 = (u64)(u32)(su >> 32) + a2.y;
+                if (nd2 < (u32)(r.state[a2.x] >> 32)) {
+                    std::printf("violated: u %u d %u -> v %u w %u d(v) %u  mask(v)=%u mask(u)=%u\n", u, (u32)(su >> 32), a2.x, a2.y,
+                                (u32)(r.state[a2.x] >> 32), (r.nearMask[0][a2.x >> 5] >> (a2.x & 31)) & 1u,
+                                (r.nearMask[0][u >> 5] >> (u & 31)) & 1u);
+                    ++shown;
+                }
             }
         }
     }

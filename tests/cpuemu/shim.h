@@ -63,6 +63,7 @@ struct EmuFiber {
 inline ucontext_t emuMain;
 inline jmp_buf emuMainJb;
 inline EmuFiber* emuCur = nullptr;
+inline std::function<void()> emuOnTimeout;   // diagnostics printed by the watchdog
 inline void emuYield() {
     if (!_setjmp(emuCur->jb)) _longjmp(emuMainJb, 1);
 }
@@ -130,6 +131,8 @@ template <class T> T __shfl_down(T v, unsigned off, int = 64) {
     const int lane = (int)(ectx.tid & 63u);
     return emuShflFrom(v, lane + (int)off < 64 ? lane + (int)off : lane);
 }
+// The wave's lanes reconverge: a barrier of the wave's fibers.
+inline void __builtin_amdgcn_wave_barrier() { emuArrive(emuWave().bar); }
 inline unsigned long long __ballot(int p) {
     EmuWave& w = emuWave();
     w.slot[ectx.tid & 63u] = p ? 1 : 0;
@@ -216,6 +219,7 @@ inline void emuRun(uint32_t grid, uint32_t block, uint32_t b0, uint32_t b1, cons
         }
         if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
             std::printf("EMU DEADLOCK/TIMEOUT: %zu lanes still live\n", live);
+            if (emuOnTimeout) emuOnTimeout();
             std::exit(3);
         }
     }

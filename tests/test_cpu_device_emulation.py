@@ -78,6 +78,8 @@ CASES = [
     ("grid", 20, "delta", "async=1 am=2 grid=4 pers=1 delta=100000"),
     ("grid", 20, "delta", "async=1 grid=4 pers=1 K=3 kmax=4 astar=1 lm=8 yield=2"),
     ("grid", 20, "delta", "async=1 grid=3 pers=1 K=3 kmax=4 astar=1 delta=300"),
+    ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=1"),
+    ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=3 seed=7"),
 ]
 
 
