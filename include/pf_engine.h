@@ -65,7 +65,8 @@ typedef struct {
     uint32_t async_light;  /* Delta-stepping: 1 = each bucket's light relaxations as one asynchronous
                               work-queue step (no grid barrier per light iteration), 0 = one step per
                               light iteration as the reference; identical results either way */
-    uint32_t reserved[2];
+    uint32_t profile_phases; /* 1 = record wall-clock time per step phase (pf_solver_phase_profile) */
+    uint32_t reserved[1];
 } pf_options;
 
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */
@@ -189,6 +190,13 @@ int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs);
  * PF_ERR_STATE until pf_serve_end. */
 int pf_serve_begin(pf_solver* s, const pf_options* o, int astar);
 int pf_serve_admit(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts, uint32_t count);
+/* Wall-clock profile of the last run made with options.profile_phases = 1: per step phase
+ * (index = the engine's phase number, 16 entries) the milliseconds from the end of the
+ * previous step to the end of each step of that phase, summed, and the number of such steps.
+ * Phases: 0 light, 1 heavy, 2 far minimum, 3 far partition, 4 frontier, 5 naive sweep,
+ * 6-8 prefix-scan relax/tile sums/scatter, 10 asynchronous light.  No reference counterpart. */
+int pf_solver_phase_profile(pf_solver* s, double* ms_out, uint32_t* steps_out);
+
 int pf_serve_run(pf_solver* s, uint32_t epoch_steps, uint32_t* done, uint32_t* idle, pf_stats* st);
 int pf_serve_astar_results(pf_solver* s, const uint32_t* slots, const uint32_t* srcs, const uint32_t* dsts,
                            uint32_t count, uint32_t* target_dists, uint32_t* paths, uint32_t path_cap,

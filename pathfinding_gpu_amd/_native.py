@@ -19,7 +19,7 @@ class PfOptions(C.Structure):
         ("algo", C.c_int32), ("mapping", C.c_int32), ("atomic", C.c_int32), ("compaction", C.c_int32),
         ("delta", C.c_uint32), ("block_size", C.c_int32), ("sync_every", C.c_uint32), ("drive", C.c_int32),
         ("wave_cutoff", C.c_uint32), ("block_cutoff", C.c_uint32), ("grid_blocks", C.c_uint32),
-        ("heuristic", C.c_int32), ("max_steps", C.c_uint32), ("async_light", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("heuristic", C.c_int32), ("max_steps", C.c_uint32), ("async_light", C.c_uint32), ("profile_phases", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -76,6 +76,7 @@ ENGINE_SYMBOLS = {
     "pf_multi_seed": (C.c_int, [C.c_void_p, u32p, C.c_uint32, C.POINTER(PfOptions), u32p, u32p,
                                 C.POINTER(PfStats)]),
     "pf_last_reach": (C.c_int, [C.c_void_p, u64p, u64p]),
+    "pf_solver_phase_profile": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "pf_debug_light_end": (C.c_int, [C.c_void_p, C.c_uint32, u32p]),
     "pf_debug_heuristic": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, u32p]),
     "pf_debug_exclusive_scan": (C.c_int, [u32p, u32p, C.c_uint32, u32p]),

@@ -447,6 +447,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         run.async = 1;
         startPhase = PH_LIGHT_ASYNC;
     }
+    run.profile = o.profile_phases ? 1u : 0u;
     if (o.algo == PF_ALGO_NAIVE) { startPhase = PH_NAIVE; mode = 1; useBound = false; }
     else if (o.algo == PF_ALGO_FRONTIER) { startPhase = scanMode ? PH_SCAN_RELAX : PH_FRONTIER; useBound = false; }
     const u64 capSteps = std::min<u64>(0xFFFFFF00ull, 16ull * Kn + 100000ull);
@@ -1315,6 +1316,21 @@ int pf_last_reach(pf_solver* s, uint64_t* reached, uint64_t* reachable_arcs) {
     PF_HIP(hipStreamSynchronize(s->s));
     if (reached) *reached = h[0];
     if (reachable_arcs) *reachable_arcs = h[1];
+    return PF_OK;
+}
+
+int pf_solver_phase_profile(pf_solver* s, double* ms_out, uint32_t* steps_out) {
+    int rc = setDev(s);
+    if (rc) return rc;
+    if (!s->lastK) return fail(PF_ERR_STATE, "no run yet");
+    unsigned long long h[2 * PROF_PHASES];
+    PF_HIP(hipMemcpyAsync(h, s->d_bar + PROF_TICKS, sizeof(h), hipMemcpyDeviceToHost, s->s));
+    PF_HIP(hipStreamSynchronize(s->s));
+    static_assert(PROF_STEPS == PROF_TICKS + PROF_PHASES, "ticks and step counts are adjacent");
+    for (u32 p = 0; p < PROF_PHASES; ++p) {
+        if (ms_out) ms_out[p] = (double)h[p] / s->wallClockHz * 1e3;
+        if (steps_out) steps_out[p] = (u32)h[PROF_PHASES + p];
+    }
     return PF_OK;
 }
 

@@ -116,6 +116,7 @@ struct Run {
     u32* ring;                   // asynchronous light phase: work queue [ringCap], RING_EMPTY when free
     u64 ringCap;
     u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
+    u32 profile;                 // accumulate wall-clock time per phase (controller, PROF_* words)
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -123,6 +124,10 @@ constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
 // buffer): pushes reserved (tail), claims taken (head), items finished (done).
 constexpr u32 RING_TAIL = 32, RING_HEAD = 48, RING_DONE = 64;
 constexpr u32 RING_EMPTY = 0xFFFFFFFFu;
+// Phase profile (barrier buffer): wall clock of the last step's end, then per phase the
+// wall-clock ticks between the previous step's end and the end of a step of that phase
+// (work, waits and the barrier) and the number of such steps.
+constexpr u32 PROF_LAST = 80, PROF_TICKS = 128, PROF_STEPS = 144, PROF_PHASES = 16;
 
 struct LocalStats {
     u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
@@ -536,71 +541,136 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
     }
 }
 
-// One lane's arcs [beg, end) in groups of four: the group's arc loads, its
-// read-before-atomic loads and its atomicMins are each issued together before
-// any result is used, so a group costs three dependent round trips instead of
-// twelve; the queue pushes of the updated arcs follow as in relaxEdge.
+// Wave-wide maximum (64 lanes).
+__device__ __forceinline__ u32 waveMax(u32 x) {
+    const u32 lane = laneId();
+#pragma unroll
+    for (u32 off = 1; off < 64; off <<= 1) {
+        const u32 y = __shfl(x, (int)(lane ^ off), 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// Pushes of one relaxation group, called by the whole wave: each lane's near items (up to four)
+// go onto the work queue (or the near queue) and its far items onto the far pile, with one
+// reservation per queue for the wave.  A work-queue slot is written only after its index has
+// come back from the tail counter, so an item is never visible before its reservation; a full
+// work queue drops the push, counts it finished and flags the run (the host then reruns it with
+// the synchronous light phase).
+__device__ __forceinline__ void pushGroup(const Run& r, const Outs& o, const u32 (&nv)[4], u32 nn,
+                                          const u32 (&fv)[4], u32 nf) {
+    const u32 lane = laneId();
+    const u32 own = nn | (nf << 16);                  // both counts in one scan (each sum <= 256)
+    const u32 incl = waveInclusiveSum(own);
+    const u32 tot = __shfl(incl, 63, 64);
+    const u32 ex = incl - own;
+    if (tot & 0xFFFFu) {
+        if (o.ring) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&r.bar[RING_TAIL], (unsigned long long)(tot & 0xFFFFu));
+            base = ((u64)__shfl((u32)(base >> 32), 0, 64) << 32) | __shfl((u32)base, 0, 64);
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q) {
+                if (q >= nn) break;
+                const u64 idx = base + (ex & 0xFFFFu) + q;
+                if (idx < r.ringCap) {
+                    stA(&r.ring[idx], nv[q]);
+                } else {
+                    atomicOr(&r.ctl->error, (u32)E_RING_FULL);
+                    atomicAdd(&r.bar[RING_DONE], 1ull);
+                }
+            }
+        } else {
+            u32 base = 0;
+            if (lane == 0) base = atomicAdd(o.nearCnt, tot & 0xFFFFu);
+            base = __shfl(base, 0, 64);
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q)
+                if (q < nn) stA(&o.nearQ[base + (ex & 0xFFFFu) + q], nv[q]);
+        }
+    }
+    if (tot >> 16) {
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(o.farCnt, tot >> 16);
+        base = __shfl(base, 0, 64);
+#pragma unroll
+        for (u32 q = 0; q < 4; ++q)
+            if (q < nf) stA(&o.farQ[base + (ex >> 16) + q], fv[q]);
+    }
+}
+
+// The lanes' own arcs [beg, end) (lanes with mine = false have none) in groups of four, called
+// by the whole wave: a group's arc loads, its read-before-atomic loads and its atomicMins are
+// each issued together before any result is used, so a group costs three dependent round trips
+// instead of twelve, and its pushes take one reservation per queue for the wave.
 template <bool ASTAR, int AM>
-__device__ __forceinline__ void relaxLane4(const Run& r, const Item& it, const Outs& o, LocalStats& ls) {
-    if constexpr (AM == AM_CAS_RBA) {
-        relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
-    } else {
-        const u32 n = r.g.n;
-        for (u32 e0 = it.beg; e0 < it.end; e0 += 4) {
-            uint2 a[4];
-            bool go[4];
-            u64 key[4], nv[4];
-            u32 gv[4], nd[4];
+__device__ __forceinline__ void relaxGroups(const Run& r, const Item& it, bool mine, const Outs& o, LocalStats& ls) {
+    static_assert(AM != AM_CAS_RBA, "the CAS loop relaxes one arc at a time");
+    const u32 n = r.g.n;
+    const u32 beg = mine ? it.beg : 0u, end = mine ? it.end : 0u;
+    const u32 len = waveMax(end - beg);
+    for (u32 off = 0; off < len; off += 4) {               // same trip count on every lane
+        const u32 e0 = beg + off;
+        uint2 a[4];
+        bool go[4];
+        u64 key[4], nv[4];
+        u32 gv[4], nd[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] = e0 + q < it.end ? r.g.adj[e0 + q] : make_uint2(0u, 0u);
+        for (int q = 0; q < 4; ++q) a[q] = e0 + q < end ? r.g.adj[e0 + q] : make_uint2(0u, 0u);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const u64 d = (u64)it.du + a[q].y;
-                const bool ok = e0 + q < it.end;
-                ls.edges += ok ? 1u : 0u;
-                go[q] = ok && d < INF;
-                key[q] = d;
-                if constexpr (ASTAR) {
-                    if (go[q]) {
-                        key[q] += heur(r, it.t, a[q].x);
-                        go[q] = key[q] <= it.ub;          // strict: ties with dist[target] stay explored
-                    }
-                }
-                nd[q] = (u32)d;
-                gv[q] = it.k * n + a[q].x;
-                nv[q] = pack(nd[q], it.p);
-            }
-            if constexpr (AM == AM_MIN_RBA) {
-                u64 cur[4];   // a stale value is >= the current one: the check only skips useless atomics
-#pragma unroll
-                for (int q = 0; q < 4; ++q) cur[q] = go[q] ? r.state[gv[q]] : 0ull;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) go[q] = go[q] && cur[q] > nv[q];
-            }
-            u64 old[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                old[q] = 0ull;
+        for (int q = 0; q < 4; ++q) {
+            const u64 d = (u64)it.du + a[q].y;
+            const bool ok = e0 + q < end;
+            ls.edges += ok ? 1u : 0u;
+            go[q] = ok && d < INF;
+            key[q] = d;
+            if constexpr (ASTAR) {
                 if (go[q]) {
-                    ls.atomics++;
-                    old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
+                    key[q] += heur(r, it.t, a[q].x);
+                    go[q] = key[q] <= it.ub;          // strict: ties with dist[target] stay explored
                 }
             }
+            nd[q] = (u32)d;
+            gv[q] = it.k * n + a[q].x;
+            nv[q] = pack(nd[q], it.p);
+        }
+        if constexpr (AM == AM_MIN_RBA) {
+            u64 cur[4];   // a stale value is >= the current one: the check only skips useless atomics
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (!(go[q] && distOf(old[q]) > nd[q])) continue;
-                ls.updates++;
-                if (key[q] < it.bnd) {
-                    if (testAndSetBit(o.nearMask, gv[q])) {
-                        if (o.ring) pushRing(r, gv[q]); else pushQ(o.nearQ, o.nearCnt, gv[q]);
-                        ls.pushes++;
-                    }
-                } else if (testAndSetBit(o.farMask, gv[q])) {
-                    pushQ(o.farQ, o.farCnt, gv[q]);
-                    ls.pushes++;
-                }
+            for (int q = 0; q < 4; ++q) cur[q] = go[q] ? r.state[gv[q]] : 0ull;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) go[q] = go[q] && cur[q] > nv[q];
+        }
+        u64 old[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            old[q] = 0ull;
+            if (go[q]) {
+                ls.atomics++;
+                old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
             }
         }
+        u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
+        u32 nn = 0, nf = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!(go[q] && distOf(old[q]) > nd[q])) continue;
+            ls.updates++;
+            if (key[q] < it.bnd) {
+                if (testAndSetBit(o.nearMask, gv[q])) {
+#pragma unroll
+                    for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gv[q];   // constant indices: registers
+                    ++nn;
+                }
+            } else if (testAndSetBit(o.farMask, gv[q])) {
+#pragma unroll
+                for (u32 j = 0; j < 4; ++j) if (j == nf) farV[j] = gv[q];
+                ++nf;
+            }
+        }
+        ls.pushes += nn + nf;
+        pushGroup(r, o, nearV, nn, farV, nf);
     }
 }
 
@@ -630,6 +700,9 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
     u64 my = NOCLAIM;
     const u32 lane = laneId();
     const unsigned long long t0 = wall_clock64();
+    u64 pending = 0;   // items this wave has finished and not yet added to the done counter
+    u32 nap = 0;       // idle back-off (s_sleep rounds)
+    bool idle = false;
     while (true) {                                        // wave-uniform
         // claims for the lanes without one: one atomic per wave
         const u64 need = __ballot(my == NOCLAIM);
@@ -643,9 +716,28 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
                 my = base + (u64)__popcll(below);
             }
         }
+        // Idle: first add the finished items to the done counter (their pushes were performed at
+        // the end of the iteration that took them), then test for the end of the phase: finished
+        // when done >= tail, done read first (here: the value the add returns).  The tail read
+        // also tells which claims have been reserved and can be polled.
+        u64 avail = ~0ull;
+        if (idle) {
+            u32 stop = 0;
+            unsigned long long t = 0;
+            if (lane == 0) {
+                const unsigned long long d = pending ? atomicAdd(done, (unsigned long long)pending) + pending : ldA(done);
+                const u32 z = zeroAfter((u32)d);
+                t = ldA(tail + z);
+                if (d >= t) stop = 1;
+                else if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); stop = 1; }
+            }
+            pending = 0;
+            if (__shfl(stop, 0, 64)) break;
+            avail = ((u64)__shfl((u32)(t >> 32), 0, 64) << 32) | __shfl((u32)t, 0, 64);
+        }
         // take the claimed item if it has arrived
         u32 gi = RING_EMPTY;
-        if (my < r.ringCap) {
+        if (my < r.ringCap && my < avail) {
             const u32 v = ldA(&r.ring[my]);
             if (v != RING_EMPTY) {
                 gi = v;
@@ -699,28 +791,28 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
                 if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
                 relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
             }
-            if (busy && !big) relaxLane4<ASTAR, AM>(r, it, o, ls);
+            if constexpr (AM == AM_CAS_RBA) {
+                if (busy && !big) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
+            } else {
+                relaxGroups<ASTAR, AM>(r, it, busy && !big, o, ls);
+            }
             // every lane of the wave is past its relaxations (the wave reconverges here), and the
-            // wave's pushes have been performed, before its items count as finished (the empty asm
-            // statements keep the compiler from moving memory operations across the wait)
+            // wave's pushes and slot resets have been performed, before its items count as finished
+            // (the empty asm statements keep the compiler from moving memory operations across the wait)
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
             waitAll();
             asm volatile("" ::: "memory");
-            if ((int)lane == __ffsll((unsigned long long)got) - 1) atomicAdd(done, (unsigned long long)__popcll(got));
+            pending += (u64)__popcll(got);
+            idle = false;
+            nap = 0;
             continue;
         }
-        // nothing taken: finished when done == tail (done read first)
-        u32 stop = 0;
-        if (lane == 0) {
-            const unsigned long long d = ldA(done);
-            const u32 z = zeroAfter((u32)d);
-            const unsigned long long t = ldA(tail + z);
-            if (d >= t) stop = 1;
-            else if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); stop = 1; }
+        if (idle) {   // nothing arrived since the last test: back off (up to ~32 x 64 clocks)
+            nap = nap < 32u ? 2u * nap + 1u : nap;
+            for (u32 i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
         }
-        if (__shfl(stop, 0, 64)) break;
-        __builtin_amdgcn_s_sleep(1);
+        idle = true;
     }
 }
 
@@ -905,6 +997,13 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         u32 next = PH_DONE;
         u32 resetMin = 0, setBounds = 0, scan = 0, resetRing = 0;
         const unsigned long long ringTail = r.async ? ldA(&r.bar[RING_TAIL]) : 0ull;
+        if (r.profile) {
+            const unsigned long long now = wall_clock64();
+            const u32 p = phase % PROF_PHASES;
+            stA(&r.bar[PROF_TICKS + p], ldA(&r.bar[PROF_TICKS + p]) + (now - ldA(&r.bar[PROF_LAST])));
+            stA(&r.bar[PROF_STEPS + p], ldA(&r.bar[PROF_STEPS + p]) + 1ull);
+            stA(&r.bar[PROF_LAST], now);
+        }
         auto bucketDone = [&]() {
             h.buckets++;
             if (h.cnt[C_FAR0 + fcur] == 0) next = PH_DONE;
@@ -1251,6 +1350,10 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         c->steps = 0; c->iters = 0; c->buckets = 0; c->error = 0;
         c->scanTotal = 0; c->epochEnd = 0; c->pause = 0; c->maxSteps = maxSteps;
         r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
+        if (r.profile) {
+            for (u32 j = 0; j < PROF_PHASES; ++j) r.bar[PROF_TICKS + j] = r.bar[PROF_STEPS + j] = 0ull;
+            r.bar[PROF_LAST] = wall_clock64();
+        }
     }
 }
 
@@ -1318,6 +1421,10 @@ __global__ void serveInitKernel(Run r) {
         r.bar[0] = 0ull;
         r.bar[BAR_GEN] = 0ull;
         r.stats[0] = r.stats[1] = r.stats[2] = r.stats[3] = r.stats[4] = 0ull;
+        if (r.profile) {
+            for (u32 j = 0; j < PROF_PHASES; ++j) r.bar[PROF_TICKS + j] = r.bar[PROF_STEPS + j] = 0ull;
+            r.bar[PROF_LAST] = wall_clock64();
+        }
     }
 }
 

@@ -73,6 +73,7 @@ class GpuOptions:
     grid_blocks: int = 0
     max_steps: int = 0
     async_light: bool = False   # Delta: each bucket's light phase as one asynchronous work-queue step
+    profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
 
     def to_c(self) -> N.PfOptions:
         o = N.PfOptions()
@@ -82,6 +83,7 @@ class GpuOptions:
         o.wave_cutoff, o.block_cutoff, o.grid_blocks = self.wave_cutoff, self.block_cutoff, self.grid_blocks
         o.heuristic, o.max_steps = int(self.heuristic), self.max_steps
         o.async_light = 1 if self.async_light else 0
+        o.profile_phases = 1 if self.profile_phases else 0
         return o
 
 
@@ -249,6 +251,18 @@ class GpuSolver:
         a, b = C.c_uint64(), C.c_uint64()
         N.check(self._lib.pf_last_reach(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    PHASES = {0: "light", 1: "heavy", 2: "far_min", 3: "far_partition", 4: "frontier", 5: "naive",
+              6: "scan_relax", 7: "scan_tiles", 8: "scan_scatter", 10: "light_async"}
+
+    def phase_profile(self) -> dict:
+        """Per step phase of the last run made with profile_phases: {name: (ms, steps)}, where ms
+        sums the wall-clock time from the end of the previous step to the end of each step of
+        that phase (its work, waits and grid barrier)."""
+        ms = (C.c_double * 16)()
+        steps = (C.c_uint32 * 16)()
+        N.check(self._lib.pf_solver_phase_profile(self._h, ms, steps))
+        return {self.PHASES.get(p, str(p)): (ms[p], steps[p]) for p in range(16) if steps[p]}
 
     # ---- test hooks ----
     def debug_light_end(self, delta: int) -> np.ndarray:

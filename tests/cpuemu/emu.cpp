@@ -171,6 +171,7 @@ int main(int argc, char** argv) {
         startPhase = PH_LIGHT_ASYNC;
     }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }
+    r.profile = gi_("prof", 0) ? 1u : 0u;
     const u32 maxSteps = 100000;
     // initKernel + seedKernel
     emuLaunch(4, 64, false, [&](u32) {
@@ -201,6 +202,11 @@ int main(int argc, char** argv) {
         }
     }
     std::printf("phase %u steps %u error %u launches %u\n", r.ctl->phase, r.ctl->steps, r.ctl->error, launches);
+    if (gi_("expect_full", 0)) {   // a work queue too small: the run must stop cleanly with E_RING_FULL
+        const bool okFull = r.ctl->phase == PH_DONE && (r.ctl->error & E_RING_FULL) && !(r.ctl->error & E_BARRIER);
+        std::printf("%s: work queue overflow %s\n", okFull ? "ok" : "FAIL", okFull ? "reported" : "not reported");
+        return okFull ? 0 : 1;
+    }
     // check
     u64 bad = 0;
     std::vector<u32> d(n), pc(n);
@@ -240,6 +246,11 @@ int main(int argc, char** argv) {
                 }
             }
         }
+    }
+    if (r.profile) {   // the phase profile counts every step once
+        unsigned long long steps = 0;
+        for (u32 p = 0; p < PROF_PHASES; ++p) steps += r.bar[PROF_STEPS + p];
+        if (steps != r.ctl->steps) { std::printf("profile counts %llu steps, run took %u\n", steps, r.ctl->steps); ++bad; }
     }
     std::printf("%s: n %u m %u algo %s map %d am %d bs %u grid %u K %u delta %u pers %d astar %u -> %llu mismatches\n",
                 bad ? "FAIL" : "ok", n, m, algo.c_str(), map, am, bs, grid, K, delta, persistent, astar,

@@ -61,7 +61,7 @@ CASES = [
     ("grid", 20, "scan", "map=2 grid=3"),
     ("rmat", 9, "scan", "map=2 grid=3 pers=1 yield=5"),
     ("grid", 20, "delta", "map=0 grid=4 pers=1 K=3 kmax=4"),
-    ("rmat", 9, "delta", "map=2 grid=3"),
+    ("rmat", 9, "delta", "map=2 grid=3 prof=1"),
     ("rmat", 9, "delta", "map=2 am=1 grid=4 pers=1 K=2 kmax=4"),
     ("grid", 20, "delta", "map=1 am=2 grid=2 pers=1"),
     ("grid", 20, "delta", "map=2 grid=4 pers=1 K=3 kmax=4 yield=3 fill=255"),
@@ -71,7 +71,7 @@ CASES = [
     ("rmat", 9, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=4 sym=0"),
     ("grid", 20, "delta", "map=0 grid=2 K=2 kmax=4 astar=1 lm=8 yield=4 delta=8000"),
     # asynchronous light phase (work queue per bucket)
-    ("grid", 20, "delta", "async=1 grid=3"),
+    ("grid", 20, "delta", "async=1 grid=3 prof=1"),
     ("grid", 20, "delta", "async=1 map=0 grid=4 pers=1 K=3 kmax=4 yield=3"),
     ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 yield=5 fill=0"),
     ("rmat", 9, "delta", "async=1 am=1 grid=3 pers=1 delta=1"),
@@ -80,6 +80,7 @@ CASES = [
     ("grid", 20, "delta", "async=1 grid=3 pers=1 K=3 kmax=4 astar=1 delta=300"),
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=1"),
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=3 seed=7"),
+    ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 delta=100000 ringcap=200 expect_full=1 yield=3"),
 ]
 
 
