@@ -66,7 +66,9 @@ typedef struct {
                               work-queue step (no grid barrier per light iteration), 0 = one step per
                               light iteration as the reference; identical results either way */
     uint32_t profile_phases; /* 1 = record wall-clock time per step phase (pf_solver_phase_profile) */
-    uint32_t reserved[1];
+    uint32_t async_backoff;  /* asynchronous light phase: longest idle wait of a wave between polls of the
+                                queue counters, in units of 64 clocks (0 = 32) */
+    uint32_t reserved[2];
 } pf_options;
 
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */

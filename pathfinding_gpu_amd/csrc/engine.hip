@@ -337,6 +337,7 @@ Run baseRun(const pf_solver* s) {
     run.invN = 1.0 / (double)s->n;
     run.bar = s->d_bar;
     run.spinTicks = (unsigned long long)(2.0 * s->wallClockHz);
+    run.napCap = 32u;
     return run;
 }
 
@@ -448,6 +449,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         startPhase = PH_LIGHT_ASYNC;
     }
     run.profile = o.profile_phases ? 1u : 0u;
+    run.napCap = o.async_backoff ? std::min<u32>(o.async_backoff, 4096u) : 32u;
     if (o.algo == PF_ALGO_NAIVE) { startPhase = PH_NAIVE; mode = 1; useBound = false; }
     else if (o.algo == PF_ALGO_FRONTIER) { startPhase = scanMode ? PH_SCAN_RELAX : PH_FRONTIER; useBound = false; }
     const u64 capSteps = std::min<u64>(0xFFFFFF00ull, 16ull * Kn + 100000ull);
@@ -923,8 +925,8 @@ int pf_solver_create(const pf_graph* g, uint32_t max_batch, pf_solver** out) {
         (rc = s->dalloc(&s->d_src, K)) || (rc = s->dalloc(&s->d_stats, 5)) ||
         (rc = s->dalloc(&s->d_reach, 2)) || (rc = s->dalloc(&s->d_lens, K)) || (rc = s->dalloc(&s->d_tdist, K)))
         return cleanupFail(rc);
-    if ((rc = s->dalloc(&s->d_ctl, 1)) || (rc = s->dalloc(&s->d_bar, 256))) return cleanupFail(rc);
-    if (hipMemset(s->d_ctl, 0, sizeof(Ctl)) != hipSuccess || hipMemset(s->d_bar, 0, 256 * 8) != hipSuccess)
+    if ((rc = s->dalloc(&s->d_ctl, 1)) || (rc = s->dalloc(&s->d_bar, BAR_WORDS))) return cleanupFail(rc);
+    if (hipMemset(s->d_ctl, 0, sizeof(Ctl)) != hipSuccess || hipMemset(s->d_bar, 0, BAR_WORDS * 8) != hipSuccess)
         return cleanupFail(fail(PF_ERR_HIP, "memset"));
     if (hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)
         return cleanupFail(fail(PF_ERR_NO_MEMORY, "pinned ctl"));

@@ -117,6 +117,7 @@ struct Run {
     u64 ringCap;
     u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
     u32 profile;                 // accumulate wall-clock time per phase (controller, PROF_* words)
+    u32 napCap;                  // async light phase: longest idle back-off, s_sleep(1) rounds
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -124,6 +125,7 @@ constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
 // buffer): pushes reserved (tail), claims taken (head), items finished (done).
 constexpr u32 RING_TAIL = 32, RING_HEAD = 48, RING_DONE = 64;
 constexpr u32 RING_EMPTY = 0xFFFFFFFFu;
+constexpr u32 BAR_WORDS = 256;   // 64-bit words of the barrier buffer
 // Phase profile (barrier buffer): wall clock of the last step's end, then per phase the
 // wall-clock ticks between the previous step's end and the end of a step of that phase
 // (work, waits and the barrier) and the number of such steps.
@@ -338,7 +340,35 @@ struct Outs {
     u32* farMask;
     u32* flags;   // prefix-scan mode: set flags[gv] instead of pushing
     u32* ring;    // asynchronous light phase: near pushes go to the work queue
+    u32* lbFar;   // workgroup buffer (LDS) of far pushes [LB_FAR], or null: push directly
+    u32* lbFarN;  // its fill count (may exceed LB_FAR: the excess was pushed directly)
 };
+
+// Workgroup push buffers (LDS) for the far pile and R: a step's far and R pushes are
+// collected per workgroup and reserved in the global queue with one atomic per workgroup at
+// the end of the step (flushBuf), instead of one per wave and relaxation round -- a single
+// queue counter serializes every atomic on it at ~80 M/s for the whole chip.  Those queues
+// are only read after the step's grid barrier.  A full buffer pushes directly.
+constexpr u32 LB_FAR = 2048, LB_R = 1024;
+__device__ __forceinline__ void pushBuf(u32* lb, u32* ln, u32 cap, u32* q, u32* qn, u32 item) {
+    const u32 i = waveAggInc(ln);
+    if (i < cap) lb[i] = item;
+    else pushQ(q, qn, item);
+}
+__device__ __forceinline__ void pushFar(const Outs& o, u32 item) {
+    if (o.lbFar) pushBuf(o.lbFar, o.lbFarN, LB_FAR, o.farQ, o.farCnt, item);
+    else pushQ(o.farQ, o.farCnt, item);
+}
+// Block-wide: appends the buffer to the global queue and empties it.
+__device__ __forceinline__ void flushBuf(u32* lb, u32* ln, u32 cap, u32* q, u32* qn, u32* s_base) {
+    __syncthreads();
+    const u32 cnt = *ln < cap ? *ln : cap;
+    if (threadIdx.x == 0 && cnt) *s_base = atomicAdd(qn, cnt);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < cnt; i += blockDim.x) stA(&q[*s_base + i], lb[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) *ln = 0u;
+}
 
 // Push onto the work queue of the asynchronous light phase.  The slot is written only after
 // its index has come back from the tail counter (address dependency), so an item is never
@@ -385,7 +415,7 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
                 ls.pushes++;
             }
         } else {
-            if (testAndSetBit(o.farMask, gv)) { pushQ(o.farQ, o.farCnt, gv); ls.pushes++; }
+            if (testAndSetBit(o.farMask, gv)) { pushFar(o, gv); ls.pushes++; }
         }
     }
 }
@@ -425,7 +455,7 @@ static_assert(sizeof(GatherSlot) == 40, "gather slot");
 // ---------------------------------------------------------------------------
 template <bool ASTAR, int AM, int MAP>
 __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMask, int sel, bool markR,
-                               bool useBound, const Outs& o, LocalStats& ls, u32* s_owner, Item* s_item,
+                               bool useBound, const Outs& o, LocalStats& ls, u32* lbR, u32* lbRN, u32* s_owner, Item* s_item,
                                GatherSlot* gather) {
     const u32 bs = blockDim.x;
     const u32 n = r.g.n;
@@ -455,7 +485,7 @@ __device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMas
                 const u32 r0 = va.x, le = va.y, r1 = r.g.vrec[it.u + 1].x;
                 it.p = va.z;
                 // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
-                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &r.ctl->cnt[C_R], gi);
+                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
                 if (sel == 0) { it.beg = r0; it.end = r1; }
                 else if (sel == 1) { it.beg = r0; it.end = le; }
                 else { it.beg = le; it.end = r1; }
@@ -591,12 +621,25 @@ __device__ __forceinline__ void pushGroup(const Run& r, const Outs& o, const u32
         }
     }
     if (tot >> 16) {
-        u32 base = 0;
-        if (lane == 0) base = atomicAdd(o.farCnt, tot >> 16);
-        base = __shfl(base, 0, 64);
+        if (o.lbFar) {   // into the workgroup buffer; what does not fit goes to the far pile directly
+            u32 base = 0;
+            if (lane == 0) base = atomicAdd(o.lbFarN, tot >> 16);
+            base = __shfl(base, 0, 64);
 #pragma unroll
-        for (u32 q = 0; q < 4; ++q)
-            if (q < nf) stA(&o.farQ[base + (ex >> 16) + q], fv[q]);
+            for (u32 q = 0; q < 4; ++q) {
+                if (q >= nf) break;
+                const u32 i = base + (ex >> 16) + q;
+                if (i < LB_FAR) o.lbFar[i] = fv[q];
+                else stA(&o.farQ[atomicAdd(o.farCnt, 1u)], fv[q]);
+            }
+        } else {
+            u32 base = 0;
+            if (lane == 0) base = atomicAdd(o.farCnt, tot >> 16);
+            base = __shfl(base, 0, 64);
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q)
+                if (q < nf) stA(&o.farQ[base + (ex >> 16) + q], fv[q]);
+        }
     }
 }
 
@@ -689,13 +732,14 @@ __device__ __forceinline__ void relaxGroups(const Run& r, const Item& it, bool m
 // min-index parent equals the synchronous phase's.
 // ---------------------------------------------------------------------------
 template <bool ASTAR, int AM>
-__device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
+__device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u32* lbFarN, u32* lbR, u32* lbRN) {
     const u32 n = r.g.n;
     Ctl* c = r.ctl;
     unsigned long long* head = &r.bar[RING_HEAD];
     unsigned long long* tail = &r.bar[RING_TAIL];
     unsigned long long* done = &r.bar[RING_DONE];
-    const Outs o{nullptr, nullptr, r.nearMask[0], r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring};
+    const Outs o{nullptr, nullptr, r.nearMask[0], r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
+                 lbFar, lbFarN};
     constexpr u64 NOCLAIM = ~0ull;
     u64 my = NOCLAIM;
     const u32 lane = laneId();
@@ -766,7 +810,7 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
                 }
                 if (active) {
                     it.p = va.z;
-                    if (va.y < r1 && testAndSetBit(r.rMask, gi)) pushQ(r.rQ, &c->cnt[C_R], gi);
+                    if (va.y < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &c->cnt[C_R], gi);
                     it.beg = va.x;
                     it.end = va.y;                       // light arcs
                     it.bnd = ldA(&r.bound[it.k]);
@@ -808,8 +852,8 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls) {
             nap = 0;
             continue;
         }
-        if (idle) {   // nothing arrived since the last test: back off (up to ~32 x 64 clocks)
-            nap = nap < 32u ? 2u * nap + 1u : nap;
+        if (idle) {   // nothing arrived since the last test: back off (up to napCap x 64 clocks)
+            nap = 2u * nap + 1u < r.napCap ? 2u * nap + 1u : r.napCap;
             for (u32 i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
         }
         idle = true;
@@ -893,7 +937,7 @@ __device__ void farMin(const Run& r, const u32* far, u32 size) {
 // Far pile partition (src/gpu_kernels.cuh:413-436).
 template <bool ASTAR>
 __device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ, u32* nearCnt, u32* nearMask,
-                             u32* farOut, u32* farOutCnt) {
+                             u32* farOut, u32* farOutCnt, u32* lbFar, u32* lbFarN) {
     const u32 n = r.g.n;
     for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size; idx += gridDim.x * blockDim.x) {
         const u32 gi = ldA(&far[idx]);
@@ -915,7 +959,7 @@ __device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ,
                 if (r.async) pushRing(r, gi); else pushQ(nearQ, nearCnt, gi);
             }
         } else {
-            pushQ(farOut, farOutCnt, gi);            // stays (bit unchanged)
+            pushBuf(lbFar, lbFarN, LB_FAR, farOut, farOutCnt, gi);   // stays (bit unchanged)
         }
     }
 }
@@ -996,7 +1040,7 @@ __device__ void controller(const Run& r, u32* waveTotals) {
         u32 in = h.in, fcur = h.fcur;
         u32 next = PH_DONE;
         u32 resetMin = 0, setBounds = 0, scan = 0, resetRing = 0;
-        const unsigned long long ringTail = r.async ? ldA(&r.bar[RING_TAIL]) : 0ull;
+        const unsigned long long queued = r.async ? ldA(&r.bar[RING_TAIL]) : 0ull;   // items queued (async light)
         if (r.profile) {
             const unsigned long long now = wall_clock64();
             const u32 p = phase % PROF_PHASES;
@@ -1028,7 +1072,7 @@ __device__ void controller(const Run& r, u32* waveTotals) {
                 h.iters++;
                 h.cnt[C_R] = 0;
                 if (r.async) {
-                    if (ringTail) next = PH_LIGHT_ASYNC;
+                    if (queued) next = PH_LIGHT_ASYNC;
                     else bucketDone();
                 } else if (h.cnt[C_NEAR0 + in]) {
                     next = PH_LIGHT;
@@ -1041,7 +1085,7 @@ __device__ void controller(const Run& r, u32* waveTotals) {
             case PH_PART:
                 h.cnt[C_FAR0 + fcur] = 0;
                 fcur ^= 1u;
-                if (r.async ? ringTail != 0ull : h.cnt[C_NEAR0 + in] != 0u) {
+                if (r.async ? queued != 0ull : h.cnt[C_NEAR0 + in] != 0u) {
                     next = r.async ? PH_LIGHT_ASYNC : PH_LIGHT;
                     if (!r.async) h.cnt[C_NEAR0 + (in ^ 1u)] = 0;
                 } else if (h.cnt[C_FAR0 + fcur]) {
@@ -1182,8 +1226,11 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
     __shared__ u32 s_owner;
     __shared__ Item s_item;
     __shared__ u32 waveTotals[17];
+    __shared__ u32 s_lbFar[LB_FAR], s_lbR[LB_R];   // workgroup push buffers (far pile, R)
+    __shared__ u32 s_lbFarN, s_lbRN, s_lbBase;
     LocalStats ls;
     Ctl* c = r.ctl;
+    if (threadIdx.x == 0) { s_lbFarN = 0u; s_lbRN = 0u; }
     while (true) {
         if (threadIdx.x == 0) {
             const u32 ph = ldAgent(&c->phase);
@@ -1204,44 +1251,52 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
         switch (phase) {
             case PH_LIGHT: {
                 const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
-                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr};
+                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, nullptr, s_lbFar, &s_lbFarN};
                 expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 1, true, true, o, ls,
-                                               &s_owner, &s_item, gather);
+                                               s_lbR, &s_lbRN, &s_owner, &s_item, gather);
                 break;
             }
             case PH_HEAVY: {
                 const Outs o{r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
-                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr};
-                expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, &s_owner, &s_item,
-                                               gather);
+                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr,
+                             s_lbFar, &s_lbFarN};
+                expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, s_lbR, &s_lbRN,
+                                               &s_owner, &s_item, gather);
                 break;
             }
             case PH_FRONTIER: {
                 const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
                              r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u], nullptr};
                 expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 0, false, false, o, ls,
-                                               &s_owner, &s_item, gather);
+                                               s_lbR, &s_lbRN, &s_owner, &s_item, gather);
                 break;
             }
             case PH_SCAN_RELAX: {
                 const Outs o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, r.flags};
-                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, nullptr, 0, false, false, o, ls, &s_owner,
+                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, nullptr, 0, false, false, o, ls, s_lbR, &s_lbRN, &s_owner,
                                                &s_item, gather);
                 break;
             }
-            case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls); break;
+            case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break;
             case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
             case PH_SCAN_SCATTER: scanScatter(r, r.nearQ[in ^ 1u], waveTotals); break;
             case PH_FARMIN: farMin<ASTAR>(r, r.farQ[fcur], size); break;
             case PH_PART:
                 farPartition<ASTAR>(r, r.farQ[fcur], size, r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
-                                    r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)]);
+                                    r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)], s_lbFar, &s_lbFarN);
                 break;
             case PH_NAIVE:
                 naiveSweep<ASTAR, AM>(r, r.act[s_cur], r.act[s_cur ^ 1u], &c->cnt[C_CHANGED], ls);
                 break;
             default: break;
         }
+        // the step's buffered far and R pushes into the global queues (block-uniform)
+        if (phase == PH_LIGHT || phase == PH_HEAVY || phase == PH_LIGHT_ASYNC)
+            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, r.farQ[fcur], &c->cnt[C_FAR0 + fcur], &s_lbBase);
+        else if (phase == PH_PART)
+            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)], &s_lbBase);
+        if (phase == PH_LIGHT || phase == PH_LIGHT_ASYNC)
+            flushBuf(s_lbR, &s_lbRN, LB_R, r.rQ, &c->cnt[C_R], &s_lbBase);
         // ---- arrive; the last workgroup runs the controller ----
         // Each wave waits until its own memory operations have completed (its atomic stores are
         // then visible device-wide); thread 0 then takes a ticket.  In persistent mode the ticket is

@@ -74,6 +74,7 @@ class GpuOptions:
     max_steps: int = 0
     async_light: bool = False   # Delta: each bucket's light phase as one asynchronous work-queue step
     profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
+    async_backoff: int = 0      # async light phase: longest idle wait between queue polls (x64 clocks; 0 = 32)
 
     def to_c(self) -> N.PfOptions:
         o = N.PfOptions()
@@ -84,6 +85,7 @@ class GpuOptions:
         o.heuristic, o.max_steps = int(self.heuristic), self.max_steps
         o.async_light = 1 if self.async_light else 0
         o.profile_phases = 1 if self.profile_phases else 0
+        o.async_backoff = self.async_backoff
         return o
 
 

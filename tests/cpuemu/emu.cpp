@@ -153,7 +153,7 @@ int main(int argc, char** argv) {
     }
     r.waveCutoff = 64; r.blockCutoff = bs;
     r.tileItems = tileItems; r.numTiles = (u32)((Kn + tileItems - 1) / tileItems); r.total = (u32)Kn;
-    r.bar = dalloc<unsigned long long>(256, true);
+    r.bar = dalloc<unsigned long long>(BAR_WORDS, true);
     r.spinTicks = 60000000000ull;   // 600 s: emulation is slow; its own watchdog catches a stall
     std::vector<u32> srcs(K);
     for (u32 k = 0; k < K; ++k) srcs[k] = (u32)((k * 2654435761ull + 7) % n);
@@ -172,6 +172,7 @@ int main(int argc, char** argv) {
     }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }
     r.profile = gi_("prof", 0) ? 1u : 0u;
+    r.napCap = (u32)gi_("nap", 32);
     const u32 maxSteps = 100000;
     // initKernel + seedKernel
     emuLaunch(4, 64, false, [&](u32) {
@@ -181,7 +182,8 @@ int main(int argc, char** argv) {
     emuLaunch((K + 127) / 128, 128, false, [&](u32) { seedKernel(r, seeds.data(), K, mode, startPhase, maxSteps, useBound); });
     emuOnTimeout = [&] {
         std::printf("ring tail %llu head %llu done %llu; ctl phase %u error %u cnt R %u far %u %u\n",
-                    (unsigned long long)r.bar[32], (unsigned long long)r.bar[48], (unsigned long long)r.bar[64],
+                    (unsigned long long)r.bar[RING_TAIL], (unsigned long long)r.bar[RING_HEAD],
+                    (unsigned long long)r.bar[RING_DONE],
                     r.ctl->phase, r.ctl->error, r.ctl->cnt[4], r.ctl->cnt[2], r.ctl->cnt[3]);
         if (r.ring) {
             int shown = 0;

@@ -3,7 +3,7 @@
     python gen_device.py NAMESPACE OUT.hpp
 
 Removes the HIP include, renames namespace pf, gives the dynamic LDS array a static
-size, and replaces the constructs that have no CPU meaning: the wave-aggregated queue
+size, shrinks the workgroup push buffers (so that both of their paths run), and replaces the constructs that have no CPU meaning: the wave-aggregated queue
 increments (one atomic per lane instead: same queue contents, another order) and
 the register-dependency zero (a fence and a zero).
 """
@@ -19,6 +19,10 @@ def generate(ns: str) -> str:
     src = src.replace("#pragma once", "").replace("#include <hip/hip_runtime.h>", "")
     src = src.replace("namespace pf {", "namespace %s {" % ns)
     src = src.replace("extern __shared__ GatherSlot gather[];", "static GatherSlot gather[1024];")
+    # small workgroup push buffers, so that small graphs fill them and take the direct-push path too
+    lb = "constexpr u32 LB_FAR = 2048, LB_R = 1024;"
+    assert lb in src
+    src = src.replace(lb, "constexpr u32 LB_FAR = 24, LB_R = 8;")
     a = src.index("__device__ __forceinline__ u32 waveAggInc(u32* ctr) {")
     b = src.index("}\n", a) + 2
     src = src[:a] + "__device__ __forceinline__ u32 waveAggInc(u32* ctr) { return atomicAdd(ctr, 1u); }\n" + src[b:]

@@ -80,7 +80,7 @@ CASES = [
     ("grid", 20, "delta", "async=1 grid=3 pers=1 K=3 kmax=4 astar=1 delta=300"),
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=1"),
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=3 seed=7"),
-    ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 delta=100000 ringcap=200 expect_full=1 yield=3"),
+    ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 delta=100000 ringcap=40 expect_full=1 yield=3"),
 ]
 
 
