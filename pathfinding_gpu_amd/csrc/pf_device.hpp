@@ -131,6 +131,11 @@ constexpr u32 BAR_WORDS = 256;   // 64-bit words of the barrier buffer
 // (work, waits and the barrier) and the number of such steps.
 constexpr u32 PROF_LAST = 80, PROF_TICKS = 128, PROF_STEPS = 144, PROF_PHASES = 16;
 
+// One of a Run's double buffers, chosen without indexing the kernel argument (a dynamic index
+// into an array member would make the compiler keep a private copy of the whole argument).
+template <class T>
+__device__ __forceinline__ T* sel2(T* const (&a)[2], u32 i) { return i ? a[1] : a[0]; }
+
 struct LocalStats {
     u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
 };
@@ -437,140 +442,6 @@ __device__ __forceinline__ u32 waveInclusiveSum(u32 x) {
     return x;
 }
 
-// Per-wave staging area of the gather tier (dynamic LDS, 64 entries per wave).
-struct GatherSlot {
-    u32 excl, u, du, k, beg;
-    u32 ubLo, ubHi, bLo, bHi;
-    u32 p;
-};
-static_assert(sizeof(GatherSlot) == 40, "gather slot");
-
-// ---------------------------------------------------------------------------
-// Frontier expansion with thread / wave / hybrid mapping.
-//   hybrid = workgroup tier (deg >= blockCutoff, owner elected through LDS,
-//            as the reference's CTA tier), wave tier (deg >= waveCutoff,
-//            64-bit ballot), and a load-balanced gather for the rest: each
-//            wave scans its lanes' degrees, stages the vertices in LDS and
-//            hands out consecutive arcs to consecutive lanes.
-// ---------------------------------------------------------------------------
-template <bool ASTAR, int AM, int MAP>
-__device__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMask, int sel, bool markR,
-                               bool useBound, const Outs& o, LocalStats& ls, u32* lbR, u32* lbRN, u32* s_owner, Item* s_item,
-                               GatherSlot* gather) {
-    const u32 bs = blockDim.x;
-    const u32 n = r.g.n;
-    for (u32 base0 = blockIdx.x * bs; base0 < size; base0 += gridDim.x * bs) {   // block-uniform
-        const u32 idx = base0 + threadIdx.x;
-        Item it;
-        it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
-        it.ub = NOKEY; it.bnd = NOKEY;
-        if (idx < size) {
-            const u32 gi = ldA(&in[idx]);
-            // The bit is cleared before the distance is read, and the read waits for the
-            // clear: a relaxation that lowers dist[gi] after the read then finds the bit
-            // clear and pushes gi again (it tests the bit only after its atomicMin).
-            const u32 z = inMask ? zeroAfter(clearBitRet(inMask, gi)) : 0u;
-            it.k = slotOf(r, gi);
-            it.u = gi - it.k * n;
-            it.du = distOf(ldA(&r.state[gi + z]));     // the latest distance: expanded with its final value
-            bool active = it.du != INF;
-            if constexpr (ASTAR) {
-                const u32 t = r.target[it.k];
-                it.t = loadTarget(r, t);
-                it.ub = distOf(r.state[it.k * n + t]);
-                if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
-            }
-            if (active) {
-                const uint4 va = r.g.vrec[it.u];
-                const u32 r0 = va.x, le = va.y, r1 = r.g.vrec[it.u + 1].x;
-                it.p = va.z;
-                // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
-                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
-                if (sel == 0) { it.beg = r0; it.end = r1; }
-                else if (sel == 1) { it.beg = r0; it.end = le; }
-                else { it.beg = le; it.end = r1; }
-                it.bnd = useBound ? ldA(&r.bound[it.k]) : NOKEY;
-            }
-        }
-        bool done = it.end <= it.beg;
-
-        if constexpr (MAP == MAP_HYBRID) {           // workgroup tier
-            while (true) {
-                if (threadIdx.x == 0) *s_owner = INF;
-                __syncthreads();
-                if (!done && it.end - it.beg >= r.blockCutoff) *s_owner = threadIdx.x;   // benign race
-                __syncthreads();
-                const u32 owner = *s_owner;
-                if (owner == INF) break;
-                if (owner == threadIdx.x) { *s_item = it; done = true; }
-                __syncthreads();
-                const Item bi = *s_item;
-                relaxRange<ASTAR, AM>(r, bi, bi.beg + threadIdx.x, bi.end, bs, o, ls);
-                __syncthreads();
-            }
-        }
-
-        if constexpr (MAP == MAP_HYBRID || MAP == MAP_WAVE) {   // wave tier
-            const u32 lane = laneId();
-            const bool want = !done && (MAP == MAP_WAVE || it.end - it.beg >= r.waveCutoff);
-            u64 ballot = __ballot(want);
-            while (ballot) {
-                const int leader = __ffsll((unsigned long long)ballot) - 1;
-                ballot &= ballot - 1;
-                Item wi;
-                wi.u = __shfl(it.u, leader, 64);
-                wi.p = __shfl(it.p, leader, 64);
-                wi.du = __shfl(it.du, leader, 64);
-                wi.k = __shfl(it.k, leader, 64);
-                wi.beg = __shfl(it.beg, leader, 64);
-                wi.end = __shfl(it.end, leader, 64);
-                wi.ub = ((u64)__shfl((u32)(it.ub >> 32), leader, 64) << 32) | __shfl((u32)it.ub, leader, 64);
-                wi.bnd = ((u64)__shfl((u32)(it.bnd >> 32), leader, 64) << 32) | __shfl((u32)it.bnd, leader, 64);
-                if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
-                relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
-            }
-            if (want) done = true;
-        }
-
-        if constexpr (MAP == MAP_THREAD) {
-            if (!done) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
-        } else if constexpr (MAP == MAP_HYBRID) {    // gather tier
-            const u32 lane = laneId();
-            const u32 deg = done ? 0u : it.end - it.beg;
-            const u32 incl = waveInclusiveSum(deg);
-            const u32 total = __shfl(incl, 63, 64);
-            GatherSlot* gs = gather + (threadIdx.x >> 6) * 64;
-            GatherSlot me;
-            me.excl = incl - deg; me.u = it.u; me.du = it.du; me.k = it.k; me.beg = it.beg;
-            me.ubLo = (u32)it.ub; me.ubHi = (u32)(it.ub >> 32);
-            me.bLo = (u32)it.bnd; me.bHi = (u32)(it.bnd >> 32);
-            me.p = it.p;
-            gs[lane] = me;
-            __syncthreads();
-            {
-                for (u32 e0 = 0; e0 < total; e0 += 64) {
-                    const u32 e = e0 + lane;
-                    if (e < total) {
-                        u32 lo = 0, hi = 63;                 // last lane with excl <= e
-                        while (lo < hi) {
-                            const u32 mid = (lo + hi + 1) >> 1;
-                            if (gs[mid].excl <= e) lo = mid; else hi = mid - 1;
-                        }
-                        const GatherSlot ow = gs[lo];
-                        Item gi;
-                        gi.u = ow.u; gi.p = ow.p; gi.du = ow.du; gi.k = ow.k;
-                        gi.ub = ((u64)ow.ubHi << 32) | ow.ubLo;
-                        gi.bnd = ((u64)ow.bHi << 32) | ow.bLo;
-                        if constexpr (ASTAR) gi.t = loadTarget(r, r.target[gi.k]);
-                        relaxEdge<ASTAR, AM>(r, gi, r.g.adj[ow.beg + (e - ow.excl)], o, ls);
-                    }
-                }
-            }
-            __syncthreads();   // the staging area is rewritten by the next chunk
-        }
-    }
-}
-
 // Wave-wide maximum (64 lanes).
 __device__ __forceinline__ u32 waveMax(u32 x) {
     const u32 lane = laneId();
@@ -643,77 +514,289 @@ __device__ __forceinline__ void pushGroup(const Run& r, const Outs& o, const u32
     }
 }
 
-// The lanes' own arcs [beg, end) (lanes with mine = false have none) in groups of four, called
-// by the whole wave: a group's arc loads, its read-before-atomic loads and its atomicMins are
-// each issued together before any result is used, so a group costs three dependent round trips
-// instead of twelve, and its pushes take one reservation per queue for the wave.
+// Four arcs of one lane, each with the context of its tail vertex (loaded by the caller).
+template <bool ASTAR>
+struct Arcs4 {
+    uint2 a[4];            // {head, weight}
+    u32 du[4], k[4], p[4]; // tail distance, slot, caller's id of the tail (the parent)
+    u64 ub[4], bnd[4];     // A* bound of the slot, bucket bound of the slot
+    bool ok[4];
+    Target t[ASTAR ? 4 : 1];
+};
+
+// Relaxes four arcs per lane, called by the whole wave: the read-before-atomic loads and the
+// atomicMins of the four arcs are each issued together before any result is used (the caller
+// issued the four arc loads together as well), so four arcs cost three dependent round trips
+// instead of twelve, and the pushes take one reservation per queue for the wave.
 template <bool ASTAR, int AM>
-__device__ __forceinline__ void relaxGroups(const Run& r, const Item& it, bool mine, const Outs& o, LocalStats& ls) {
+__device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, const Outs& o, LocalStats& ls) {
     static_assert(AM != AM_CAS_RBA, "the CAS loop relaxes one arc at a time");
     const u32 n = r.g.n;
-    const u32 beg = mine ? it.beg : 0u, end = mine ? it.end : 0u;
-    const u32 len = waveMax(end - beg);
-    for (u32 off = 0; off < len; off += 4) {               // same trip count on every lane
-        const u32 e0 = beg + off;
-        uint2 a[4];
-        bool go[4];
-        u64 key[4], nv[4];
-        u32 gv[4], nd[4];
+    bool go[4];
+    u64 key[4], nv[4];
+    u32 gv[4], nd[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = e0 + q < end ? r.g.adj[e0 + q] : make_uint2(0u, 0u);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const u64 d = (u64)it.du + a[q].y;
-            const bool ok = e0 + q < end;
-            ls.edges += ok ? 1u : 0u;
-            go[q] = ok && d < INF;
-            key[q] = d;
-            if constexpr (ASTAR) {
-                if (go[q]) {
-                    key[q] += heur(r, it.t, a[q].x);
-                    go[q] = key[q] <= it.ub;          // strict: ties with dist[target] stay explored
-                }
-            }
-            nd[q] = (u32)d;
-            gv[q] = it.k * n + a[q].x;
-            nv[q] = pack(nd[q], it.p);
-        }
-        if constexpr (AM == AM_MIN_RBA) {
-            u64 cur[4];   // a stale value is >= the current one: the check only skips useless atomics
-#pragma unroll
-            for (int q = 0; q < 4; ++q) cur[q] = go[q] ? r.state[gv[q]] : 0ull;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) go[q] = go[q] && cur[q] > nv[q];
-        }
-        u64 old[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            old[q] = 0ull;
+    for (int q = 0; q < 4; ++q) {
+        const u64 d = (u64)x.du[q] + x.a[q].y;
+        ls.edges += x.ok[q] ? 1u : 0u;
+        go[q] = x.ok[q] && d < INF;
+        key[q] = d;
+        if constexpr (ASTAR) {
             if (go[q]) {
-                ls.atomics++;
-                old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
+                key[q] += heur(r, x.t[q], x.a[q].x);
+                go[q] = key[q] <= x.ub[q];            // strict: ties with dist[target] stay explored
             }
         }
-        u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
-        u32 nn = 0, nf = 0;
+        nd[q] = (u32)d;
+        gv[q] = x.k[q] * n + x.a[q].x;
+        nv[q] = pack(nd[q], x.p[q]);
+    }
+    if constexpr (AM == AM_MIN_RBA) {
+        u64 cur[4];   // a stale value is >= the current one: the check only skips useless atomics
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (!(go[q] && distOf(old[q]) > nd[q])) continue;
-            ls.updates++;
-            if (key[q] < it.bnd) {
-                if (testAndSetBit(o.nearMask, gv[q])) {
+        for (int q = 0; q < 4; ++q) cur[q] = go[q] ? r.state[gv[q]] : 0ull;
 #pragma unroll
-                    for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gv[q];   // constant indices: registers
-                    ++nn;
+        for (int q = 0; q < 4; ++q) go[q] = go[q] && cur[q] > nv[q];
+    }
+    u64 old[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        old[q] = 0ull;
+        if (go[q]) {
+            ls.atomics++;
+            old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
+        }
+    }
+    u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
+    u32 nn = 0, nf = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!(go[q] && distOf(old[q]) > nd[q])) continue;
+        ls.updates++;
+        if (o.flags) {
+            stA(&o.flags[gv[q]], 1u);
+            ls.pushes++;
+        } else if (key[q] < x.bnd[q]) {
+            if (testAndSetBit(o.nearMask, gv[q])) {
+#pragma unroll
+                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gv[q];   // constant indices: registers
+                ++nn;
+            }
+        } else if (testAndSetBit(o.farMask, gv[q])) {
+#pragma unroll
+            for (u32 j = 0; j < 4; ++j) if (j == nf) farV[j] = gv[q];
+            ++nf;
+        }
+    }
+    ls.pushes += nn + nf;
+    pushGroup(r, o, nearV, nn, farV, nf);
+}
+
+// One vertex's arcs [beg, end) spread over a wave or a workgroup: the caller's lane `me` of
+// `stride` relaxes beg + me + stride * j, four per round.  Called by every lane of the wave or
+// workgroup with the same item: the number of rounds is the same on every lane.
+template <bool ASTAR, int AM>
+__device__ __forceinline__ void relaxSpan4(const Run& r, const Item& it, u32 beg, u32 end, u32 stride, u32 me,
+                                           const Outs& o, LocalStats& ls) {
+    if constexpr (AM == AM_CAS_RBA) {
+        relaxRange<ASTAR, AM>(r, it, beg + me, end, stride, o, ls);
+    } else {
+        for (u32 b = beg; b < end; b += 4 * stride) {
+            Arcs4<ASTAR> x;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u32 e = b + (u32)q * stride + me;
+                x.ok[q] = e < end;
+                x.a[q] = x.ok[q] ? r.g.adj[e] : make_uint2(0u, 0u);
+                x.du[q] = it.du; x.k[q] = it.k; x.p[q] = it.p; x.ub[q] = it.ub; x.bnd[q] = it.bnd;
+                if constexpr (ASTAR) x.t[q] = it.t;
+            }
+            relaxCore4<ASTAR, AM>(r, x, o, ls);
+        }
+    }
+}
+
+// The lanes' own arcs [beg, end) (lanes with mine = false have none), four consecutive arcs
+// per round, called by the whole wave.
+template <bool ASTAR, int AM>
+__device__ __forceinline__ void relaxGroups(const Run& r, const Item& it, bool mine, const Outs& o, LocalStats& ls) {
+    if constexpr (AM == AM_CAS_RBA) {
+        if (mine) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
+    } else {
+        const u32 beg = mine ? it.beg : 0u, end = mine ? it.end : 0u;
+        const u32 len = waveMax(end - beg);
+        for (u32 off = 0; off < len; off += 4) {               // same trip count on every lane
+            Arcs4<ASTAR> x;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u32 e = beg + off + (u32)q;
+                x.ok[q] = e < end;
+                x.a[q] = x.ok[q] ? r.g.adj[e] : make_uint2(0u, 0u);
+                x.du[q] = it.du; x.k[q] = it.k; x.p[q] = it.p; x.ub[q] = it.ub; x.bnd[q] = it.bnd;
+                if constexpr (ASTAR) x.t[q] = it.t;
+            }
+            relaxCore4<ASTAR, AM>(r, x, o, ls);
+        }
+    }
+}
+
+// Per-wave staging area of the gather tier (dynamic LDS, 64 entries per wave).
+struct GatherSlot {
+    u32 excl, u, du, k, beg;
+    u32 ubLo, ubHi, bLo, bHi;
+    u32 p;
+};
+static_assert(sizeof(GatherSlot) == 40, "gather slot");
+
+// ---------------------------------------------------------------------------
+// Frontier expansion with thread / wave / hybrid mapping.
+//   hybrid = workgroup tier (deg >= blockCutoff, owner elected through LDS,
+//            as the reference's CTA tier), wave tier (deg >= waveCutoff,
+//            64-bit ballot), and a load-balanced gather for the rest: each
+//            wave scans its lanes' degrees, stages the vertices in LDS and
+//            hands out consecutive arcs to consecutive lanes.
+// ---------------------------------------------------------------------------
+template <bool ASTAR, int AM, int MAP>
+__device__ __forceinline__ void expandFrontier(const Run& r, const u32* in, u32 size, u32* inMask, int sel, bool markR,
+                               bool useBound, const Outs& o, LocalStats& ls, u32* lbR, u32* lbRN, u32* s_owner, Item* s_item,
+                               GatherSlot* gather) {
+    const u32 bs = blockDim.x;
+    const u32 n = r.g.n;
+    for (u32 base0 = blockIdx.x * bs; base0 < size; base0 += gridDim.x * bs) {   // block-uniform
+        const u32 idx = base0 + threadIdx.x;
+        Item it;
+        it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
+        it.ub = NOKEY; it.bnd = NOKEY;
+        if (idx < size) {
+            const u32 gi = ldA(&in[idx]);
+            // The bit is cleared before the distance is read, and the read waits for the
+            // clear: a relaxation that lowers dist[gi] after the read then finds the bit
+            // clear and pushes gi again (it tests the bit only after its atomicMin).
+            const u32 z = inMask ? zeroAfter(clearBitRet(inMask, gi)) : 0u;
+            it.k = slotOf(r, gi);
+            it.u = gi - it.k * n;
+            it.du = distOf(ldA(&r.state[gi + z]));     // the latest distance: expanded with its final value
+            bool active = it.du != INF;
+            if constexpr (ASTAR) {
+                const u32 t = r.target[it.k];
+                it.t = loadTarget(r, t);
+                it.ub = distOf(r.state[it.k * n + t]);
+                if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
+            }
+            if (active) {
+                const uint4 va = r.g.vrec[it.u];
+                const u32 r0 = va.x, le = va.y, r1 = r.g.vrec[it.u + 1].x;
+                it.p = va.z;
+                // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
+                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
+                if (sel == 0) { it.beg = r0; it.end = r1; }
+                else if (sel == 1) { it.beg = r0; it.end = le; }
+                else { it.beg = le; it.end = r1; }
+                it.bnd = useBound ? ldA(&r.bound[it.k]) : NOKEY;
+            }
+        }
+        bool done = it.end <= it.beg;
+
+        if constexpr (MAP == MAP_HYBRID) {           // workgroup tier
+            while (true) {
+                if (threadIdx.x == 0) *s_owner = INF;
+                __syncthreads();
+                if (!done && it.end - it.beg >= r.blockCutoff) *s_owner = threadIdx.x;   // benign race
+                __syncthreads();
+                const u32 owner = *s_owner;
+                if (owner == INF) break;
+                if (owner == threadIdx.x) { *s_item = it; done = true; }
+                __syncthreads();
+                const Item bi = *s_item;
+                relaxSpan4<ASTAR, AM>(r, bi, bi.beg, bi.end, bs, threadIdx.x, o, ls);
+                __syncthreads();
+            }
+        }
+
+        if constexpr (MAP == MAP_HYBRID || MAP == MAP_WAVE) {   // wave tier
+            const u32 lane = laneId();
+            const bool want = !done && (MAP == MAP_WAVE || it.end - it.beg >= r.waveCutoff);
+            u64 ballot = __ballot(want);
+            while (ballot) {
+                const int leader = __ffsll((unsigned long long)ballot) - 1;
+                ballot &= ballot - 1;
+                Item wi;
+                wi.u = __shfl(it.u, leader, 64);
+                wi.p = __shfl(it.p, leader, 64);
+                wi.du = __shfl(it.du, leader, 64);
+                wi.k = __shfl(it.k, leader, 64);
+                wi.beg = __shfl(it.beg, leader, 64);
+                wi.end = __shfl(it.end, leader, 64);
+                wi.ub = ((u64)__shfl((u32)(it.ub >> 32), leader, 64) << 32) | __shfl((u32)it.ub, leader, 64);
+                wi.bnd = ((u64)__shfl((u32)(it.bnd >> 32), leader, 64) << 32) | __shfl((u32)it.bnd, leader, 64);
+                if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
+                relaxSpan4<ASTAR, AM>(r, wi, wi.beg, wi.end, 64u, lane, o, ls);
+            }
+            if (want) done = true;
+        }
+
+        if constexpr (MAP == MAP_THREAD) {
+            relaxGroups<ASTAR, AM>(r, it, !done, o, ls);
+        } else if constexpr (MAP == MAP_HYBRID) {    // gather tier
+            const u32 lane = laneId();
+            const u32 deg = done ? 0u : it.end - it.beg;
+            const u32 incl = waveInclusiveSum(deg);
+            const u32 total = __shfl(incl, 63, 64);
+            GatherSlot* gs = gather + (threadIdx.x >> 6) * 64;
+            GatherSlot me;
+            me.excl = incl - deg; me.u = it.u; me.du = it.du; me.k = it.k; me.beg = it.beg;
+            me.ubLo = (u32)it.ub; me.ubHi = (u32)(it.ub >> 32);
+            me.bLo = (u32)it.bnd; me.bHi = (u32)(it.bnd >> 32);
+            me.p = it.p;
+            gs[lane] = me;
+            __syncthreads();
+            if constexpr (AM == AM_CAS_RBA) {
+                for (u32 e0 = 0; e0 < total; e0 += 64) {
+                    const u32 e = e0 + lane;
+                    if (e < total) {
+                        u32 lo = 0, hi = 63;                 // last lane with excl <= e
+                        while (lo < hi) {
+                            const u32 mid = (lo + hi + 1) >> 1;
+                            if (gs[mid].excl <= e) lo = mid; else hi = mid - 1;
+                        }
+                        const GatherSlot ow = gs[lo];
+                        Item gi;
+                        gi.u = ow.u; gi.p = ow.p; gi.du = ow.du; gi.k = ow.k;
+                        gi.ub = ((u64)ow.ubHi << 32) | ow.ubLo;
+                        gi.bnd = ((u64)ow.bHi << 32) | ow.bLo;
+                        if constexpr (ASTAR) gi.t = loadTarget(r, r.target[gi.k]);
+                        relaxEdge<ASTAR, AM>(r, gi, r.g.adj[ow.beg + (e - ow.excl)], o, ls);
+                    }
                 }
-            } else if (testAndSetBit(o.farMask, gv[q])) {
+            } else {
+                // four arcs per lane and round (arcs e0 + 64 q + lane): the four owner searches run
+                // interleaved (fixed six steps: the last lane with excl <= e, never a lane with no
+                // arcs), then the four arc loads, then the relaxations
+                for (u32 e0 = 0; e0 < total; e0 += 256) {   // wave-uniform
+                    u32 e[4], lo[4];
 #pragma unroll
-                for (u32 j = 0; j < 4; ++j) if (j == nf) farV[j] = gv[q];
-                ++nf;
+                    for (int q = 0; q < 4; ++q) { e[q] = e0 + 64u * (u32)q + lane; lo[q] = 0u; }
+#pragma unroll
+                    for (u32 step = 32; step > 0; step >>= 1) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) lo[q] += gs[lo[q] + step].excl <= e[q] ? step : 0u;
+                    }
+                    Arcs4<ASTAR> x;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const GatherSlot ow = gs[lo[q]];
+                        x.ok[q] = e[q] < total;
+                        x.a[q] = x.ok[q] ? r.g.adj[ow.beg + (e[q] - ow.excl)] : make_uint2(0u, 0u);
+                        x.du[q] = ow.du; x.k[q] = ow.k; x.p[q] = ow.p;
+                        x.ub[q] = ((u64)ow.ubHi << 32) | ow.ubLo;
+                        x.bnd[q] = ((u64)ow.bHi << 32) | ow.bLo;
+                        if constexpr (ASTAR) x.t[q] = loadTarget(r, r.target[ow.k]);
+                    }
+                    relaxCore4<ASTAR, AM>(r, x, o, ls);
+                }
             }
+            __syncthreads();   // the staging area is rewritten by the next chunk
         }
-        ls.pushes += nn + nf;
-        pushGroup(r, o, nearV, nn, farV, nf);
     }
 }
 
@@ -732,13 +815,13 @@ __device__ __forceinline__ void relaxGroups(const Run& r, const Item& it, bool m
 // min-index parent equals the synchronous phase's.
 // ---------------------------------------------------------------------------
 template <bool ASTAR, int AM>
-__device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u32* lbFarN, u32* lbR, u32* lbRN) {
+__device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u32* lbFarN, u32* lbR, u32* lbRN) {
     const u32 n = r.g.n;
     Ctl* c = r.ctl;
     unsigned long long* head = &r.bar[RING_HEAD];
     unsigned long long* tail = &r.bar[RING_TAIL];
     unsigned long long* done = &r.bar[RING_DONE];
-    const Outs o{nullptr, nullptr, r.nearMask[0], r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
+    const Outs o{nullptr, nullptr, r.nearMask[0], sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
                  lbFar, lbFarN};
     constexpr u64 NOCLAIM = ~0ull;
     u64 my = NOCLAIM;
@@ -833,7 +916,7 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u
                 wi.ub = ((u64)__shfl((u32)(it.ub >> 32), leader, 64) << 32) | __shfl((u32)it.ub, leader, 64);
                 wi.bnd = ((u64)__shfl((u32)(it.bnd >> 32), leader, 64) << 32) | __shfl((u32)it.bnd, leader, 64);
                 if constexpr (ASTAR) wi.t = loadTarget(r, r.target[wi.k]);
-                relaxRange<ASTAR, AM>(r, wi, wi.beg + lane, wi.end, 64u, o, ls);
+                relaxSpan4<ASTAR, AM>(r, wi, wi.beg, wi.end, 64u, lane, o, ls);
             }
             if constexpr (AM == AM_CAS_RBA) {
                 if (busy && !big) relaxRange<ASTAR, AM>(r, it, it.beg, it.end, 1u, o, ls);
@@ -862,7 +945,7 @@ __device__ void asyncLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u
 
 // Naive synchronous relaxation, one thread per vertex (src/gpu_kernels.cuh:297-329).
 template <bool ASTAR, int AM>
-__device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, LocalStats& ls) {
+__device__ __forceinline__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, LocalStats& ls) {
     const u32 n = r.g.n;
     Target t;
     u64 ubState = NOKEY;
@@ -901,7 +984,7 @@ __device__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, Lo
 
 // Far pile minimum (src/gpu_kernels.cuh:386-410), per slot.
 template <bool ASTAR>
-__device__ void farMin(const Run& r, const u32* far, u32 size) {
+__device__ __forceinline__ void farMin(const Run& r, const u32* far, u32 size) {
     const u32 n = r.g.n;
     for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx - threadIdx.x < size; idx += gridDim.x * blockDim.x) {
         unsigned long long key = NOKEY;
@@ -936,7 +1019,7 @@ __device__ void farMin(const Run& r, const u32* far, u32 size) {
 
 // Far pile partition (src/gpu_kernels.cuh:413-436).
 template <bool ASTAR>
-__device__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ, u32* nearCnt, u32* nearMask,
+__device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ, u32* nearCnt, u32* nearMask,
                              u32* farOut, u32* farOutCnt, u32* lbFar, u32* lbFarN) {
     const u32 n = r.g.n;
     for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size; idx += gridDim.x * blockDim.x) {
@@ -987,7 +1070,7 @@ __device__ __forceinline__ u32 blockExclusiveScan(u32 x, u32* waveTotals, u32& e
     return tot;
 }
 
-__device__ void scanTiles(const Run& r, u32* waveTotals) {
+__device__ __forceinline__ void scanTiles(const Run& r, u32* waveTotals) {
     const u32 per = r.tileItems / blockDim.x;
     for (u32 t = blockIdx.x; t < r.numTiles; t += gridDim.x) {
         const u32 b = t * r.tileItems;
@@ -1002,7 +1085,7 @@ __device__ void scanTiles(const Run& r, u32* waveTotals) {
     }
 }
 
-__device__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
+__device__ __forceinline__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
     const u32 per = r.tileItems / blockDim.x;
     for (u32 t = blockIdx.x; t < r.numTiles; t += gridDim.x) {
         const u32 b = t * r.tileItems;
@@ -1029,7 +1112,7 @@ __device__ void scanScatter(const Run& r, u32* outQ, u32* waveTotals) {
 // arrived.  It reads the control block with a few wide loads into LDS, decides
 // the next phase there, and writes the block back with a few wide stores, so its
 // critical path is a handful of memory round trips.
-__device__ void controller(const Run& r, u32* waveTotals) {
+__device__ __forceinline__ void controller(const Run& r, u32* waveTotals) {
     Ctl* c = r.ctl;
     __shared__ __align__(16) Ctl h;           // working copy
     __shared__ u32 s_next, s_resetMin, s_setBounds, s_scan, s_part;
@@ -1250,51 +1333,51 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
         if (phase == PH_DONE) break;
         switch (phase) {
             case PH_LIGHT: {
-                const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
-                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, nullptr, s_lbFar, &s_lbFarN};
-                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 1, true, true, o, ls,
+                const Outs o{sel2(r.nearQ, in ^ 1u), &c->cnt[C_NEAR0 + (in ^ 1u)], sel2(r.nearMask, in ^ 1u),
+                             sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, nullptr, s_lbFar, &s_lbFarN};
+                expandFrontier<ASTAR, AM, MAP>(r, sel2(r.nearQ, in), size, sel2(r.nearMask, in), 1, true, true, o, ls,
                                                s_lbR, &s_lbRN, &s_owner, &s_item, gather);
                 break;
             }
             case PH_HEAVY: {
-                const Outs o{r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
-                             r.farQ[fcur], &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr,
+                const Outs o{sel2(r.nearQ, in), &c->cnt[C_NEAR0 + in], sel2(r.nearMask, in),
+                             sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr,
                              s_lbFar, &s_lbFarN};
                 expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, s_lbR, &s_lbRN,
                                                &s_owner, &s_item, gather);
                 break;
             }
             case PH_FRONTIER: {
-                const Outs o{r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u],
-                             r.nearQ[in ^ 1u], &c->cnt[C_NEAR0 + (in ^ 1u)], r.nearMask[in ^ 1u], nullptr};
-                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, r.nearMask[in], 0, false, false, o, ls,
+                const Outs o{sel2(r.nearQ, in ^ 1u), &c->cnt[C_NEAR0 + (in ^ 1u)], sel2(r.nearMask, in ^ 1u),
+                             sel2(r.nearQ, in ^ 1u), &c->cnt[C_NEAR0 + (in ^ 1u)], sel2(r.nearMask, in ^ 1u), nullptr};
+                expandFrontier<ASTAR, AM, MAP>(r, sel2(r.nearQ, in), size, sel2(r.nearMask, in), 0, false, false, o, ls,
                                                s_lbR, &s_lbRN, &s_owner, &s_item, gather);
                 break;
             }
             case PH_SCAN_RELAX: {
                 const Outs o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, r.flags};
-                expandFrontier<ASTAR, AM, MAP>(r, r.nearQ[in], size, nullptr, 0, false, false, o, ls, s_lbR, &s_lbRN, &s_owner,
+                expandFrontier<ASTAR, AM, MAP>(r, sel2(r.nearQ, in), size, nullptr, 0, false, false, o, ls, s_lbR, &s_lbRN, &s_owner,
                                                &s_item, gather);
                 break;
             }
             case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break;
             case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
-            case PH_SCAN_SCATTER: scanScatter(r, r.nearQ[in ^ 1u], waveTotals); break;
-            case PH_FARMIN: farMin<ASTAR>(r, r.farQ[fcur], size); break;
+            case PH_SCAN_SCATTER: scanScatter(r, sel2(r.nearQ, in ^ 1u), waveTotals); break;
+            case PH_FARMIN: farMin<ASTAR>(r, sel2(r.farQ, fcur), size); break;
             case PH_PART:
-                farPartition<ASTAR>(r, r.farQ[fcur], size, r.nearQ[in], &c->cnt[C_NEAR0 + in], r.nearMask[in],
-                                    r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)], s_lbFar, &s_lbFarN);
+                farPartition<ASTAR>(r, sel2(r.farQ, fcur), size, sel2(r.nearQ, in), &c->cnt[C_NEAR0 + in], sel2(r.nearMask, in),
+                                    sel2(r.farQ, fcur ^ 1u), &c->cnt[C_FAR0 + (fcur ^ 1u)], s_lbFar, &s_lbFarN);
                 break;
             case PH_NAIVE:
-                naiveSweep<ASTAR, AM>(r, r.act[s_cur], r.act[s_cur ^ 1u], &c->cnt[C_CHANGED], ls);
+                naiveSweep<ASTAR, AM>(r, sel2(r.act, s_cur), sel2(r.act, s_cur ^ 1u), &c->cnt[C_CHANGED], ls);
                 break;
             default: break;
         }
         // the step's buffered far and R pushes into the global queues (block-uniform)
         if (phase == PH_LIGHT || phase == PH_HEAVY || phase == PH_LIGHT_ASYNC)
-            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, r.farQ[fcur], &c->cnt[C_FAR0 + fcur], &s_lbBase);
+            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], &s_lbBase);
         else if (phase == PH_PART)
-            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, r.farQ[fcur ^ 1u], &c->cnt[C_FAR0 + (fcur ^ 1u)], &s_lbBase);
+            flushBuf(s_lbFar, &s_lbFarN, LB_FAR, sel2(r.farQ, fcur ^ 1u), &c->cnt[C_FAR0 + (fcur ^ 1u)], &s_lbBase);
         if (phase == PH_LIGHT || phase == PH_LIGHT_ASYNC)
             flushBuf(s_lbR, &s_lbRN, LB_R, r.rQ, &c->cnt[C_R], &s_lbBase);
         // ---- arrive; the last workgroup runs the controller ----
@@ -1448,7 +1531,7 @@ __global__ void admitKernel(Run r, const u32* slots, const u32* srcs, const u32*
         r.slotDone[k] = 0u;
         const u32 fcur = c->fcur;
         atomicOr(r.farMask + (gi >> 5), 1u << (gi & 31u));
-        r.farQ[fcur][atomicAdd(&c->cnt[C_FAR0 + fcur], 1u)] = gi;
+        sel2(r.farQ, fcur)[atomicAdd(&c->cnt[C_FAR0 + fcur], 1u)] = gi;
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_wasDone) {
