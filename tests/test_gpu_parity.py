@@ -363,3 +363,21 @@ def test_async_light_full_size_grid(orc, grid1000):
         want = orc.multi_source(g.row_offsets, g.col_indices, g.weights, srcs, 0)[0]
         r = sv.multi_source(srcs, P.GpuOptions(drive=DR.Persistent, delta=auto, async_light=True))
         assert np.array_equal(r.dist, want.reshape(16, g.n))
+
+
+@pytest.mark.parametrize("async_light", [False, True])
+def test_phase_profile(orc, grid100, sgrid100, async_light):
+    """The per-phase wall-clock profile counts every step of the run once, under the phase
+    that ran it; profiling leaves the results unchanged."""
+    o = P.GpuOptions(algo=GA.DeltaStep, drive=DR.Persistent, async_light=async_light, profile_phases=True)
+    r = sgrid100.sssp(4321, o)
+    assert_sssp_exact(orc, grid100, 4321, r, f"profiled async={async_light}")
+    prof = sgrid100.phase_profile()
+    assert sum(n for _, n in prof.values()) == r.stats.steps
+    assert all(ms >= 0.0 for ms, _ in prof.values())
+    light = "light_async" if async_light else "light"
+    assert light in prof and prof[light][1] >= 1
+    assert set(prof) <= {light, "heavy", "far_min", "far_partition"}
+    # host drive: one step per launch, counted the same way
+    r = sgrid100.sssp(4321, P.GpuOptions(algo=GA.DeltaStep, drive=DR.Host, async_light=async_light, profile_phases=True))
+    assert sum(n for _, n in sgrid100.phase_profile().values()) == r.stats.steps
