@@ -154,7 +154,7 @@ def main():
         def tfo(o):
             return sv.multi_source(srcs, o, copy_back=False).ms
 
-        tuned = autotune(g, K, pickK["delta"], info["cu_count"], tfo, passes=1, reps=1, cache=args.tune_db)
+        tuned = autotune(g, K, pickK["delta"], info["cu_count"], tfo, passes=2, reps=1, cache=args.tune_db)
         cand = tuned.options(info["cu_count"], delta=pickK["delta"])
         if tf_batch_o(cand) < tf_batch(pickK["delta"], pickK["mapping"]):
             learnedK = cand
@@ -163,7 +163,7 @@ def main():
         def tfo1(o):
             return sv.sssp(src0, o, copy_back=False).ms
 
-        tuned1 = autotune(g, 1, pick1["delta"], info["cu_count"], tfo1, passes=1, reps=2, cache=args.tune_db)
+        tuned1 = autotune(g, 1, pick1["delta"], info["cu_count"], tfo1, passes=2, reps=2, cache=args.tune_db)
         cand1 = tuned1.options(info["cu_count"], delta=pick1["delta"])
         if min(tfo1(cand1) for _ in range(2)) < tf_single(pick1["delta"], pick1["mapping"]):
             learned1 = cand1

@@ -74,6 +74,8 @@ typedef struct {
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */
 typedef struct {
     uint64_t edges_scanned, atomics_issued, updates, pushes;
+    uint64_t queue_atomics;   /* atomics on the asynchronous light phase's work-queue head and tail
+                                 counters (claims and reservations, one per wave and round) */
     uint32_t iterations, host_syncs, launches, buckets, delta, steps;
     double ms;       /* init + loop (hipEvents on the engine stream) */
     double loop_ms;  /* the step launches only (after init and seeding) */

@@ -26,7 +26,7 @@ class PfOptions(C.Structure):
 class PfStats(C.Structure):
     _fields_ = [
         ("edges_scanned", C.c_uint64), ("atomics_issued", C.c_uint64), ("updates", C.c_uint64),
-        ("pushes", C.c_uint64), ("iterations", C.c_uint32), ("host_syncs", C.c_uint32),
+        ("pushes", C.c_uint64), ("queue_atomics", C.c_uint64), ("iterations", C.c_uint32), ("host_syncs", C.c_uint32),
         ("launches", C.c_uint32), ("buckets", C.c_uint32), ("delta", C.c_uint32), ("steps", C.c_uint32),
         ("ms", C.c_double), ("loop_ms", C.c_double),
     ]

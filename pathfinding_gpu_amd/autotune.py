@@ -31,13 +31,15 @@ from .solver import Drive, GpuOptions, Mapping
 
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pathfinding_gpu_amd", "autotune.json")
 
+# Searched in this order: the structural choices first (mapping, light phase), whose best launch
+# shape differs, then the launch shape.
 SPACE = {
+    "mapping": (0, 1, 2),
+    "async_light": (0, 1),
+    "grid_mult": (0.5, 1, 2, 4),
     "block_size": (128, 256, 512, 1024),
     "wave_cutoff": (16, 32, 64, 128),
     "block_cutoff_mult": (1, 4, 16),
-    "grid_mult": (0.5, 1, 2, 4),
-    "mapping": (0, 1, 2),
-    "async_light": (0, 1),
 }
 
 
@@ -60,7 +62,7 @@ class Tuned:
 
 
 def signature(g: CSRGraph, batch: int, delta: int) -> str:
-    return f"v2:{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
+    return f"v3:{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
 
 
 def autotune(g: CSRGraph, batch: int, delta: int, cus: int, time_fn: Callable[[GpuOptions], float],

@@ -556,6 +556,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         st->atomics_issued = hs[1];
         st->updates = hs[2];
         st->pushes = hs[3];
+        st->queue_atomics = hs[4];
         st->iterations = hc.iters;
         st->host_syncs = syncs;
         st->launches = launches;

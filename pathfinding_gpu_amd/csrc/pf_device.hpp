@@ -138,6 +138,7 @@ __device__ __forceinline__ T* sel2(T* const (&a)[2], u32 i) { return i ? a[1] : 
 
 struct LocalStats {
     u32 edges = 0, atomics = 0, updates = 0, pushes = 0;
+    u32 queueOps = 0;   // atomics on the work-queue head and tail counters (asynchronous light phase)
 };
 
 // ---------------------------------------------------------------------------
@@ -589,6 +590,8 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
         }
     }
     ls.pushes += nn + nf;
+    const u64 pushing = __ballot(nn != 0);                              // the whole wave takes part
+    if (o.ring && laneId() == 0 && pushing) ls.queueOps++;              // the wave's tail reservation
     pushGroup(r, o, nearV, nn, farV, nf);
 }
 
@@ -836,7 +839,10 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
         if (need) {
             const int first = __ffsll((unsigned long long)need) - 1;
             unsigned long long base = 0;
-            if ((int)lane == first) base = atomicAdd(head, (unsigned long long)__popcll(need));
+            if ((int)lane == first) {
+                base = atomicAdd(head, (unsigned long long)__popcll(need));
+                ls.queueOps++;
+            }
             base = ((u64)__shfl((u32)(base >> 32), first, 64) << 32) | __shfl((u32)base, first, 64);
             if (my == NOCLAIM) {
                 const u64 below = need & ((1ull << lane) - 1ull);
@@ -1285,19 +1291,21 @@ __device__ __forceinline__ void controller(const Run& r, u32* waveTotals) {
 // the step kernel
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void flushStats(unsigned long long* stats, const LocalStats& ls) {
-    unsigned long long a = ls.edges, b = ls.atomics, c = ls.updates, d = ls.pushes;
+    unsigned long long a = ls.edges, b = ls.atomics, c = ls.updates, d = ls.pushes, e = ls.queueOps;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off, 64);
         b += __shfl_down(b, off, 64);
         c += __shfl_down(c, off, 64);
         d += __shfl_down(d, off, 64);
+        e += __shfl_down(e, off, 64);
     }
     if (laneId() == 0) {
         if (a) atomicAdd(&stats[0], a);
         if (b) atomicAdd(&stats[1], b);
         if (c) atomicAdd(&stats[2], c);
         if (d) atomicAdd(&stats[3], d);
+        if (e) atomicAdd(&stats[4], e);
     }
 }
 

@@ -102,11 +102,12 @@ class GpuStats:
     delta: int = 0
     steps: int = 0
     loop_ms: float = 0.0
+    queue_atomics: int = 0
 
     @staticmethod
     def from_c(s: N.PfStats) -> "GpuStats":
         return GpuStats(s.edges_scanned, s.atomics_issued, s.updates, s.pushes, s.iterations, s.host_syncs,
-                        s.launches, s.buckets, s.delta, s.steps, s.loop_ms)
+                        s.launches, s.buckets, s.delta, s.steps, s.loop_ms, s.queue_atomics)
 
 
 @dataclass
