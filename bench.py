@@ -84,6 +84,14 @@ def main():
     ap.add_argument("--rmat-slots", type=int, default=24)
     ap.add_argument("--epoch", type=int, default=48)
     ap.add_argument("--landmarks", type=int, default=8, help="ALT landmarks for the serving A* (0 = floor(Euclid))")
+    ap.add_argument("--grid-mapping", type=int, default=0,
+                    help="mapping of the serving grid engine (0 thread, 1 wave, 2 hybrid; -1 = the network's pick)")
+    ap.add_argument("--grid-delta-mult", type=float, default=16.0,
+                    help="serving grid Delta with landmarks, in multiples of the Davidson default")
+    ap.add_argument("--grid-blocks", type=int, default=256, help="workgroups of the serving grid engine (0 = automatic)")
+    ap.add_argument("--rmat-mapping", type=int, default=-1,
+                    help="mapping of the serving R-MAT engine (-1 = the network's pick)")
+    ap.add_argument("--rmat-blocks", type=int, default=0, help="workgroups of the serving R-MAT engine (0 = automatic)")
     args = ap.parse_args()
     if args.serve:
         return serve(args)
@@ -252,6 +260,19 @@ def main():
         pa = np.stack([rng.integers(0, g.n, QA), rng.integers(0, g.n, QA)], 1).astype(np.uint32)
         pickA = L.DeltaModel.load().choose(g, QA) if tuner.model is not None else {"delta": auto, "mapping": 2}
         aopt = opt(pickA["delta"], pickA["mapping"], heuristic=P.Heuristic.FloorEuclid)
+        if not args.no_tune:   # launch shape of the A* batch at the learned Delta (kept only if faster)
+            from pathfinding_gpu_amd.autotune import autotune
+
+            def tfe(o_):
+                return sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)["ms"]
+
+            e_tuned = autotune(g, QA, int(pickA["delta"]), info["cu_count"], tfe, passes=2, reps=1,
+                               cache=args.tune_db, kind="astar-euclid")
+            cand = e_tuned.options(info["cu_count"], delta=int(pickA["delta"]), heuristic=P.Heuristic.FloorEuclid)
+            tfe(cand)
+            if min(tfe(cand) for _ in range(2)) < min(tfe(aopt) for _ in range(2)):
+                aopt = cand
+            log(f"[bench] autotuned (A* floor Euclid): {e_tuned}")
         sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192)
         a_ts, _ = timed(lambda: sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192), max(2, args.steps // 3))
         a_ms = float(np.mean(a_ts))
@@ -261,15 +282,29 @@ def main():
         p_ts, _ = timed(lambda: [sv.astar(int(pa[i, 0]), int(pa[i, 1]), popt, path_cap=8192) for i in range(nq)], 1)
         lm_ids, _ = sv.build_landmarks(8)
         # Delta for the landmark heuristic: f = d + h is nearly constant along shortest paths, so
-        # wider buckets pay off; judged once by timing among 1x, 4x and 8x the Davidson default
+        # wider buckets pay off; judged once by timing among 1x, 4x, 8x and 16x the Davidson default
         alt_ms = {}
-        for mult in (1, 4, 8):
+        for mult in (1, 4, 8, 16):
             for asy in (False, True):
                 o_ = opt(auto * mult, pickA["mapping"], heuristic=P.Heuristic.ALT, async_light=asy)
                 sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)
                 alt_ms[(mult, asy)] = sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)["ms"]
         alt_mult, alt_async = min(alt_ms, key=alt_ms.get)
         lopt = opt(auto * alt_mult, pickA["mapping"], heuristic=P.Heuristic.ALT, async_light=alt_async)
+        alt_tuned = None
+        if not args.no_tune:   # launch shape of the ALT batch at that Delta (kept only if faster)
+            from pathfinding_gpu_amd.autotune import autotune
+
+            def tfa(o_):
+                return sv.astar_batch(pa[:, 0], pa[:, 1], o_, path_cap=8192)["ms"]
+
+            alt_tuned = autotune(g, QA, int(auto * alt_mult), info["cu_count"], tfa, passes=2, reps=1,
+                                 cache=args.tune_db, kind="astar-alt")
+            cand = alt_tuned.options(info["cu_count"], delta=int(auto * alt_mult), heuristic=P.Heuristic.ALT)
+            tfa(cand)
+            if min(tfa(cand) for _ in range(2)) < min(tfa(lopt) for _ in range(2)):
+                lopt = cand
+            log(f"[bench] autotuned (A* ALT): {alt_tuned}")
         ra = sv.astar_batch(pa[:, 0], pa[:, 1], lopt, path_cap=8192)
         re_ = sv.astar_batch(pa[:, 0], pa[:, 1], aopt, path_cap=8192)
         alt_same = bool(np.array_equal(ra["dist"], re_["dist"]) and
@@ -277,7 +312,8 @@ def main():
         l_ts, _ = timed(lambda: sv.astar_batch(pa[:, 0], pa[:, 1], lopt, path_cap=8192), max(2, args.steps // 3))
         l_ms = float(np.mean(l_ts))
         out["astar_alt"] = {"queries": QA, "landmarks": len(lm_ids), "delta": int(auto * alt_mult),
-                            "async_light": alt_async,
+                            "async_light": bool(lopt.async_light), "mapping": int(lopt.mapping),
+                            "grid_blocks": lopt.grid_blocks, "block_size": lopt.block_size,
                             "judged_ms": {f"{m}x{'/async' if a else ''}": round(v, 2) for (m, a), v in alt_ms.items()},
                             "batch_ms": round(l_ms, 3),
                             "queries_per_s": round(QA / (l_ms * 1e-3), 1),
@@ -287,7 +323,8 @@ def main():
         out["astar"] = {"queries": QA, "batch_ms": round(a_ms, 3), "queries_per_s": round(QA / (a_ms * 1e-3), 1),
                         "latency_ms_p50": round(float(np.percentile(a_ts, 50)), 3),
                         "latency_ms_p99": round(float(np.percentile(a_ts, 99)), 3),
-                        "delta": pickA["delta"], "mapping": pickA["mapping"],
+                        "delta": pickA["delta"], "mapping": int(aopt.mapping), "grid_blocks": aopt.grid_blocks,
+                        "block_size": aopt.block_size, "async_light": bool(aopt.async_light),
                         "plain_one_at_a_time_queries_per_s": round(nq / (p_ts[0] * 1e-3), 1),
                         "note": "all queries of a batch complete together: latency = batch time"}
 
@@ -420,15 +457,21 @@ def serve(args):
     model = L.DeltaModel.load()
     pg = model.choose(grid, args.grid_slots)
     pr = model.choose(rmat, args.rmat_slots)
+    if args.rmat_mapping >= 0:
+        pr = {**pr, "mapping": args.rmat_mapping}
     heur = P.Heuristic.ALT if args.landmarks else P.Heuristic.FloorEuclid
     if args.landmarks:   # wider buckets for the landmark bound (f nearly constant along shortest paths)
-        pg = {**pg, "delta": 8 * L.davidson_delta(grid.avg_w, grid.avg_deg)}
+        pg = {**pg, "delta": int(args.grid_delta_mult * L.davidson_delta(grid.avg_w, grid.avg_deg))}
+    if args.grid_mapping >= 0:   # the A* batch sweeps on the 1M grid: thread mapping, 256 workgroups fastest
+        pg = {**pg, "mapping": args.grid_mapping}
     eng = {
         "grid": SlotEngine(grid, args.grid_slots, P.GpuOptions(drive=P.Drive.Persistent, delta=pg["delta"],
-                                                               mapping=P.Mapping(pg["mapping"]), heuristic=heur),
+                                                               mapping=P.Mapping(pg["mapping"]), heuristic=heur,
+                                                               grid_blocks=args.grid_blocks),
                            astar=True, landmarks=args.landmarks),
         "rmat": SlotEngine(rmat, args.rmat_slots, P.GpuOptions(drive=P.Drive.Persistent, delta=pr["delta"],
-                                                               mapping=P.Mapping(pr["mapping"])), astar=False,
+                                                               mapping=P.Mapping(pr["mapping"]),
+                                                               grid_blocks=args.rmat_blocks), astar=False,
                            keep_sssp=False),
     }
     reqs = poisson_requests(args.serve_rate, args.serve_requests, {"grid": 0.9, "rmat": 0.1},
@@ -449,8 +492,9 @@ def serve(args):
         "config": {"workload": "grid 2000x2000 seed 3 (90% A*) + rmat s22 ef16 seed 22 (10% SSSP), "
                                "Poisson arrivals", "grid_slots": args.grid_slots, "rmat_slots": args.rmat_slots,
                    "epoch_steps": args.epoch, "grid_delta": pg["delta"], "grid_mapping": pg["mapping"],
+                   "grid_blocks": args.grid_blocks,
                    "astar_heuristic": heur.name, "landmarks": args.landmarks,
-                   "rmat_delta": pr["delta"], "rmat_mapping": pr["mapping"]},
+                   "rmat_delta": pr["delta"], "rmat_mapping": pr["mapping"], "rmat_blocks": args.rmat_blocks},
         "dtype": "u32", "data": "synthetic",
     }
     for e in eng.values():

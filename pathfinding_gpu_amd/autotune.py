@@ -61,14 +61,16 @@ class Tuned:
                           async_light=bool(self.async_light), **kw)
 
 
-def signature(g: CSRGraph, batch: int, delta: int) -> str:
-    return f"v3:{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
+def signature(g: CSRGraph, batch: int, delta: int, kind: str = "") -> str:
+    """Database key: graph shape, batch size and Delta; `kind` names another workload on the same
+    graph (e.g. "astar-alt" for A* batches with the landmark heuristic)."""
+    return f"v3{':' + kind if kind else ''}:{g.n}:{g.m}:{g.max_deg}:{g.avg_w:.3f}:{batch}:{delta}"
 
 
 def autotune(g: CSRGraph, batch: int, delta: int, cus: int, time_fn: Callable[[GpuOptions], float],
-             passes: int = 2, reps: int = 2, cache: Optional[str] = CACHE, log=None) -> Tuned:
+             passes: int = 2, reps: int = 2, cache: Optional[str] = CACHE, log=None, kind: str = "") -> Tuned:
     """time_fn(options) -> milliseconds of one representative run (options carry delta)."""
-    key = signature(g, batch, delta)
+    key = signature(g, batch, delta, kind)
     db: Dict[str, dict] = {}
     if cache and os.path.exists(cache):
         try:
