@@ -89,9 +89,9 @@ def main():
     ap.add_argument("--grid-delta-mult", type=float, default=16.0,
                     help="serving grid Delta with landmarks, in multiples of the Davidson default")
     ap.add_argument("--grid-blocks", type=int, default=256, help="workgroups of the serving grid engine (0 = automatic)")
-    ap.add_argument("--rmat-mapping", type=int, default=-1,
-                    help="mapping of the serving R-MAT engine (-1 = the network's pick)")
-    ap.add_argument("--rmat-blocks", type=int, default=0, help="workgroups of the serving R-MAT engine (0 = automatic)")
+    ap.add_argument("--rmat-mapping", type=int, default=2,
+                    help="mapping of the serving R-MAT engine (-1 = the network's pick; hybrid measured fastest)")
+    ap.add_argument("--rmat-blocks", type=int, default=1024, help="workgroups of the serving R-MAT engine (0 = automatic)")
     args = ap.parse_args()
     if args.serve:
         return serve(args)
