@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--rmat-mapping", type=int, default=2,
                     help="mapping of the serving R-MAT engine (-1 = the network's pick; hybrid measured fastest)")
     ap.add_argument("--rmat-blocks", type=int, default=1024, help="workgroups of the serving R-MAT engine (0 = automatic)")
+    ap.add_argument("--concurrent", action="store_true",
+                    help="drive the two serving engines from two host threads (their kernels overlap on the GPU)")
     args = ap.parse_args()
     if args.serve:
         return serve(args)
@@ -478,7 +480,7 @@ def serve(args):
                             {"grid": grid.n, "rmat": rmat.n}, {"grid": True, "rmat": False}, seed=5)
     warm = poisson_requests(args.serve_rate, 64, {"grid": 0.9, "rmat": 0.1}, {"grid": grid.n, "rmat": rmat.n},
                             {"grid": True, "rmat": False}, seed=6)
-    srv = Server(eng, epoch_steps=args.epoch)
+    srv = Server(eng, epoch_steps=args.epoch, concurrent=args.concurrent)
     srv.run(warm, time_limit_s=120, log=log)
     rep = srv.run(reqs, time_limit_s=600, log=log)
     astar_done = [r for r in reqs if r.graph == "grid"]
@@ -492,7 +494,7 @@ def serve(args):
         "config": {"workload": "grid 2000x2000 seed 3 (90% A*) + rmat s22 ef16 seed 22 (10% SSSP), "
                                "Poisson arrivals", "grid_slots": args.grid_slots, "rmat_slots": args.rmat_slots,
                    "epoch_steps": args.epoch, "grid_delta": pg["delta"], "grid_mapping": pg["mapping"],
-                   "grid_blocks": args.grid_blocks,
+                   "grid_blocks": args.grid_blocks, "concurrent_engines": args.concurrent,
                    "astar_heuristic": heur.name, "landmarks": args.landmarks,
                    "rmat_delta": pr["delta"], "rmat_mapping": pr["mapping"], "rmat_blocks": args.rmat_blocks},
         "dtype": "u32", "data": "synthetic",

@@ -15,6 +15,7 @@ until a slot -- i.e. K*n*8 bytes of state -- is free.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -136,13 +137,78 @@ class ServeReport:
 
 
 class Server:
-    """Continuous batching over several resident graphs."""
+    """Continuous batching over several resident graphs.
 
-    def __init__(self, engines: Dict[str, SlotEngine], epoch_steps: int = 64):
+    concurrent=True drives each graph's engine from a host thread of its own, so the graphs' epochs
+    (persistent kernels on the engines' own streams) run on the GPU at the same time instead of one
+    after the other.  Each persistent kernel needs all of its workgroups resident at once, so this
+    requires every engine to have an explicit workgroup count (options.grid_blocks) and the engines
+    together to stay within `resident_threads` (default: three quarters of four 256-thread
+    workgroups per CU, the occupancy of the step kernel)."""
+
+    def __init__(self, engines: Dict[str, SlotEngine], epoch_steps: int = 64, concurrent: bool = False,
+                 resident_threads: Optional[int] = None):
         self.engines = engines
         self.epoch_steps = epoch_steps
+        self.concurrent = concurrent
+        if concurrent:
+            if resident_threads is None:
+                from .solver import device_info
+                resident_threads = 3 * device_info(0)["cu_count"] * 256
+            need = 0
+            for k, e in engines.items():
+                if e.opt.grid_blocks <= 0:
+                    raise ValueError(f"concurrent serving needs an explicit grid_blocks for engine {k!r}")
+                need += e.opt.grid_blocks * e.opt.block_size
+            if need > resident_threads:
+                raise ValueError(f"concurrent serving: engines need {need} resident threads, limit {resident_threads}")
+
+    def _serve_graph(self, e: SlotEngine, reqs: List[Request], t0: float, time_limit_s: float,
+                     finished: List[Request], errors: List[BaseException]) -> None:
+        """One engine's loop (concurrent mode): admit its arrived requests, run an epoch, collect."""
+        stream = deque(sorted(reqs, key=lambda r: r.arrival))
+        pending: deque = deque()
+        try:
+            while stream or pending or e.busy:
+                now = time.perf_counter() - t0
+                if now > time_limit_s:
+                    raise TimeoutError(f"serving did not finish within {time_limit_s}s")
+                while stream and stream[0].arrival <= now:
+                    pending.append(stream.popleft())
+                take = []
+                while pending and e.free and len(take) < len(e.free):
+                    take.append(pending.popleft())
+                e.admit(take, now)
+                if e.busy:
+                    for r in e.epoch(self.epoch_steps):
+                        r.done = time.perf_counter() - t0
+                        finished.append(r)
+                elif stream:
+                    wait = stream[0].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.002))
+        except BaseException as ex:   # reported by run() after every thread has ended
+            errors.append(ex)
+
+    def _run_concurrent(self, requests: List[Request], time_limit_s: float) -> tuple:
+        t0 = time.perf_counter()
+        finished: List[Request] = []
+        errors: List[BaseException] = []
+        threads = [threading.Thread(target=self._serve_graph,
+                                    args=(e, [r for r in requests if r.graph == k], t0, time_limit_s, finished, errors))
+                   for k, e in self.engines.items()]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return finished, time.perf_counter() - t0
 
     def run(self, requests: List[Request], time_limit_s: float = 120.0, log=None) -> ServeReport:
+        if self.concurrent:
+            finished, wall = self._run_concurrent(requests, time_limit_s)
+            return self._report(finished, wall)
         pending = {k: deque() for k in self.engines}
         stream = deque(sorted(requests, key=lambda r: r.arrival))
         finished: List[Request] = []
@@ -176,6 +242,9 @@ class Server:
                 log(f"[serve] t={now:.1f}s finished={len(finished)} in-flight="
                     f"{sum(len(e.busy) for e in self.engines.values())} queued={sum(len(q) for q in pending.values())}")
         wall = time.perf_counter() - t0
+        return self._report(finished, wall)
+
+    def _report(self, finished: List[Request], wall: float) -> ServeReport:
         lat = np.array([(r.done - r.arrival) * 1e3 for r in finished])
         rep = ServeReport(len(finished), wall, len(finished) / wall,
                           {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),

@@ -13,10 +13,11 @@ def graphs():
     return {"grid": P.grid_road(60, 60, 3), "rmat": P.rmat(11, 16, 5)}
 
 
-def _engines(graphs, slots=(6, 3)):
-    o = P.GpuOptions(drive=P.Drive.Persistent, heuristic=P.Heuristic.FloorEuclid)
+def _engines(graphs, slots=(6, 3), grid_blocks=0):
+    o = P.GpuOptions(drive=P.Drive.Persistent, heuristic=P.Heuristic.FloorEuclid, grid_blocks=grid_blocks)
     return {"grid": SlotEngine(graphs["grid"], slots[0], o, astar=True),
-            "rmat": SlotEngine(graphs["rmat"], slots[1], P.GpuOptions(drive=P.Drive.Persistent), astar=False)}
+            "rmat": SlotEngine(graphs["rmat"], slots[1], P.GpuOptions(drive=P.Drive.Persistent, grid_blocks=grid_blocks),
+                               astar=False)}
 
 
 def _check_oracle(orc, graphs, r):
@@ -85,3 +86,38 @@ def test_serving_blocks_other_runs(graphs):
             e.sv.sssp(0, P.GpuOptions())
     finally:
         e.close()
+
+
+def test_concurrent_engines_match_oracle(orc, graphs):
+    """Each graph's engine driven from its own host thread (both persistent kernels on the GPU at
+    once): every answer still equals the oracle's."""
+    reqs = poisson_requests(3000.0, 40, {"grid": 0.75, "rmat": 0.25}, {k: g.n for k, g in graphs.items()},
+                            {"grid": True, "rmat": False}, seed=12)
+    eng = _engines(graphs, grid_blocks=64)
+    try:
+        rep = Server(eng, epoch_steps=3, concurrent=True).run(reqs, time_limit_s=60)
+    finally:
+        for e in eng.values():
+            e.close()
+    assert rep.requests == len(reqs)
+    assert set(rep.per_graph) == {"grid", "rmat"}
+    for r in reqs:
+        assert r.done >= r.admitted >= r.arrival
+        _check_oracle(orc, graphs, r)
+
+
+def test_concurrent_engines_need_explicit_resident_grids(graphs):
+    eng = _engines(graphs)
+    try:
+        with pytest.raises(ValueError):
+            Server(eng, concurrent=True)
+    finally:
+        for e in eng.values():
+            e.close()
+    eng = _engines(graphs, grid_blocks=512)
+    try:
+        with pytest.raises(ValueError):
+            Server(eng, concurrent=True, resident_threads=512 * 256)
+    finally:
+        for e in eng.values():
+            e.close()
