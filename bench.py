@@ -412,6 +412,16 @@ def bench_union(args):
         sv.multi_seed(seeds, o, copy_back=False)
         ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
         res[name] = (float(np.mean(ts)), r)
+    if not args.no_tune:   # launch shape (mapping, light phase, workgroups) at the learned Delta
+        from pathfinding_gpu_amd.autotune import autotune
+        cus = P.device_info(0)["cu_count"]
+        tuned = autotune(u, 1, int(pick["delta"]), cus, lambda o_: sv.multi_seed(seeds, o_, copy_back=False).ms,
+                         passes=2, reps=1, cache=args.tune_db, kind="union512")
+        o = tuned.options(cus, delta=int(pick["delta"]))
+        sv.multi_seed(seeds, o, copy_back=False)
+        ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
+        res["persistent_tuned"] = (float(np.mean(ts)), r)
+        log(f"[union] autotuned: {tuned}")
     reached, arcs = sv.last_reach()
     best = min(res, key=lambda k: res[k][0])
     ms = res[best][0]
