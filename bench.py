@@ -35,14 +35,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TUNE_DB = os.path.join(ROOT, "pathfinding_gpu_amd", "data", "autotune.json")
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stepkernel_grid1000_k64_hbm_traffic.md")
+TRAFFIC_PROFILES = {"grid1000": os.path.join(ROOT, "profiles", "stepkernel_grid1000_k64_hbm_traffic.md"),
+                    "rmat20": os.path.join(ROOT, "profiles", "stepkernel_rmat20_k64_hbm_traffic.md")}
 
 
-def measured_traffic(opts: dict):
+def measured_traffic(opts: dict, workload: str = "grid1000"):
     """HBM bytes per launch of the step kernel from the committed rocprofv3 counter summary,
     if it was measured with exactly these launch options (else None)."""
     try:
-        for line in open(TRAFFIC_PROFILE):
+        for line in open(TRAFFIC_PROFILES[workload]):
             if line.startswith("traffic: "):
                 rec = json.loads(line[len("traffic: "):])
                 if rec.get("options") == opts:
@@ -209,7 +210,7 @@ def main():
     timed_opts = {"delta": int(learnedK.delta), "mapping": int(learnedK.mapping), "block_size": int(learnedK.block_size),
                   "wave_cutoff": int(learnedK.wave_cutoff), "block_cutoff": int(learnedK.block_cutoff),
                   "grid_blocks": int(learnedK.grid_blocks), "async_light": int(learnedK.async_light)}
-    traffic = measured_traffic(timed_opts) if args.workload == "grid1000" and K == 64 else None
+    traffic = measured_traffic(timed_opts, args.workload) if K == 64 else None
 
     out = {
         "metric": "GTEPS, learned-Delta batched (K=64 SSSP per launch) vs plain default-Delta single query; "

@@ -10,7 +10,8 @@ Run (one counter per rocprofv3 run, on the GPU):
 runs, in order: one streaming copy of 1 GiB (counter calibration: 1 GiB read,
 1 GiB written), then `--steps` batched runs of K=64 single-source Delta-stepping
 on the 1000x1000 grid road graph (seed 7; vertex 0 + 63 random sources, the
-benchmark's step), with the persistent kernel and the launch options given as
+benchmark's step) or, with `--workload rmat20`, on the scale-20 R-MAT (hub + 63
+random sources), with the persistent kernel and the launch options given as
 JSON (the benchmark prints the ones it timed), and prints the reach statistics
 that give the step kernel's algorithmic bytes.
 
@@ -39,12 +40,19 @@ BATCH = 64
 COPY_BYTES = 1 << 30
 
 
-def workload(batch: int = BATCH):
-    from .graph import grid_road
-    g = grid_road(1000, 1000, 7)
+def workload(batch: int = BATCH, name: str = "grid1000"):
+    """The benchmark's batch: the 1000x1000 grid (seed 7) from vertex 0, or the scale-20 R-MAT
+    (edge factor 16, seed 11) from its highest-degree vertex, plus batch-1 random sources."""
+    from .graph import grid_road, rmat
+    if name == "rmat20":
+        g = rmat(20, 16, 11)
+        src0 = g.max_degree_vertex()
+    else:
+        g = grid_road(1000, 1000, 7)
+        src0 = 0
     rng = np.random.default_rng(2026)
     cand = np.flatnonzero(np.diff(g.row_offsets) > 0)
-    srcs = np.concatenate([[0], rng.choice(cand[cand != 0], size=batch - 1, replace=False)]).astype(np.uint32)
+    srcs = np.concatenate([[src0], rng.choice(cand[cand != src0], size=batch - 1, replace=False)]).astype(np.uint32)
     return g, srcs
 
 
@@ -61,7 +69,7 @@ def run(a):
     from .solver import GpuSolver
     ms = C.c_double()
     N.check(N.engine().pf_debug_copy(COPY_BYTES, 1, C.byref(ms)))
-    g, srcs = workload(a.batch)
+    g, srcs = workload(a.batch, a.workload)
     sv = GpuSolver(g, max_batch=a.batch)
     opts = json.loads(a.options) if a.options else {"delta": sv.auto_delta(), "mapping": 2}
     o = options_from(opts)
@@ -136,6 +144,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--workload", default="grid1000", choices=["grid1000", "rmat20"])
     ap.add_argument("--options", default="", help="JSON launch options of the batched run "
                                                   "(delta, mapping, block_size, wave_cutoff, block_cutoff, grid_blocks)")
     ap.add_argument("--summarize", default="", help="directory holding FETCH_SIZE/ and WRITE_SIZE/ results")
