@@ -381,3 +381,31 @@ def test_phase_profile(orc, grid100, sgrid100, async_light):
     # host drive: one step per launch, counted the same way
     r = sgrid100.sssp(4321, P.GpuOptions(algo=GA.DeltaStep, drive=DR.Host, async_light=async_light, profile_phases=True))
     assert sum(n for _, n in sgrid100.phase_profile().values()) == r.stats.steps
+
+
+@pytest.mark.parametrize("workload", ["grid1000", "rmat20"])
+def test_benchmark_configurations_full_size(orc, workload):
+    """The benchmark's timed launch at full size: the K=64 batch (its sources) with the Delta and the
+    launch shape the shipped tuning database holds for it; every slot's dist and parent equal the
+    oracle's."""
+    import json
+    import os
+
+    from pathfinding_gpu_amd.autotune import Tuned, signature
+    from pathfinding_gpu_amd.profile_run import workload as bench_batch
+
+    g, srcs = bench_batch(64, workload)
+    db = json.load(open(os.path.join(os.path.dirname(P.__file__), "data", "autotune.json")))
+    delta = 4076 if workload == "grid1000" else 32        # the learned Delta the benchmark times
+    key = signature(g, 64, delta)
+    assert key in db, key
+    t = Tuned(**db[key])
+    o = t.options(P.device_info(0)["cu_count"], delta=delta)
+    with P.GpuSolver(g, max_batch=64) as sv:
+        r = sv.multi_source(srcs, o, parents=True)
+    dist = r.dist.reshape(64, g.n)
+    par = r.parent.reshape(64, g.n)
+    for k, s in enumerate(srcs):
+        d, pc = oracle_sssp(orc, g, s)
+        assert np.array_equal(dist[k], d), (workload, k, int(s))
+        assert np.array_equal(par[k], pc), (workload, k, int(s))
