@@ -989,67 +989,135 @@ __device__ __forceinline__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, 
 }
 
 // Far pile minimum (src/gpu_kernels.cuh:386-410), per slot.
+// Key of far-pile item gi for the partition and the minimum: dist (+ h for A*), or NOKEY when the
+// item is stale (unreached, pruned by the A* bound, or below the slot's old bound: already settled
+// in that bucket).  The state read is device-coherent: the latest distance.
 template <bool ASTAR>
-__device__ __forceinline__ void farMin(const Run& r, const u32* far, u32 size) {
-    const u32 n = r.g.n;
-    for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx - threadIdx.x < size; idx += gridDim.x * blockDim.x) {
-        unsigned long long key = NOKEY;
-        u32 k = 0;
-        if (idx < size) {
-            const u32 gi = ldA(&far[idx]);
-            k = slotOf(r, gi);
-            const u32 d = distOf(ldA(&r.state[gi]));
-            if (d != INF) {
-                u64 kk = d;
-                bool ok = true;
-                if constexpr (ASTAR) {
-                    const u32 t = r.target[k];
-                    kk += heur(r, loadTarget(r, t), gi - k * n);
-                    ok = kk <= (u64)distOf(r.state[k * n + t]);
-                }
-                if (ok && kk >= ldA(&r.bound[k])) key = kk;   // below the old bound: already settled in that bucket
-            }
+__device__ __forceinline__ unsigned long long farKey(const Run& r, u32 gi, u32 k, u32 d) {
+    if (d == INF) return NOKEY;
+    u64 key = d;
+    if constexpr (ASTAR) {
+        const u32 t = r.target[k];
+        key += heur(r, loadTarget(r, t), gi - k * r.g.n);
+        if (key > (u64)distOf(r.state[k * r.g.n + t])) return NOKEY;
+    }
+    return key >= ldA(&r.bound[k]) ? key : NOKEY;
+}
+
+// Per-slot minimum key of the far pile (src/gpu_kernels.cuh:388-410).  Four items per thread and
+// round are loaded together (far entries, then their states).  Several slots: each workgroup
+// keeps its slots' minima in LDS (`mins`, K <= LB_FAR / 2 entries; the far-push buffer, idle in
+// this phase) and applies them to the global minima once at the end; one slot: a wave minimum,
+// one atomic per wave.
+template <bool ASTAR>
+__device__ __forceinline__ void farMin(const Run& r, const u32* far, u32 size, unsigned long long* mins) {
+    const bool local = r.K > 1 && r.K <= LB_FAR / 2;
+    if (local) {
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) mins[k] = NOKEY;
+        __syncthreads();
+    }
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 - threadIdx.x < size; i0 += 4 * stride) {
+        u32 gi[4], k[4], d[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gi[q] = i0 + q * stride < size ? ldA(&far[i0 + q * stride]) : 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            k[q] = slotOf(r, gi[q]);
+            d[q] = i0 + q * stride < size ? distOf(ldA(&r.state[gi[q]])) : INF;
         }
+        unsigned long long key[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) key[q] = farKey<ASTAR>(r, gi[q], k[q], d[q]);
         if (r.K == 1) {                              // one slot: wave minimum, one atomic per wave
+            unsigned long long m = key[0];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) m = key[q] < m ? key[q] : m;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
-                const unsigned long long o2 = __shfl_down(key, off, 64);
-                key = o2 < key ? o2 : key;
+                const unsigned long long o2 = __shfl_down(m, off, 64);
+                m = o2 < m ? o2 : m;
             }
-            if (laneId() == 0 && key != NOKEY) atomicMin(&r.minKey[0], key);
-        } else if (key != NOKEY && key < ldA(&r.minKey[k])) {
-            atomicMin(&r.minKey[k], key);
+            if (laneId() == 0 && m != NOKEY) atomicMin(&r.minKey[0], m);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (key[q] == NOKEY) continue;
+                if (local) atomicMin(&mins[k[q]], key[q]);
+                else if (key[q] < ldA(&r.minKey[k[q]])) atomicMin(&r.minKey[k[q]], key[q]);
+            }
+        }
+    }
+    if (local) {
+        __syncthreads();
+        for (u32 k = threadIdx.x; k < r.K; k += blockDim.x) {
+            const unsigned long long m = mins[k];
+            if (m != NOKEY && m < ldA(&r.minKey[k])) atomicMin(&r.minKey[k], m);
         }
     }
 }
 
-// Far pile partition (src/gpu_kernels.cuh:413-436).
+// Far pile partition (src/gpu_kernels.cuh:413-436), four items per thread and round: the far
+// entries, their states, the new bounds, the mask clears, the near-mask checks and sets are each
+// issued for the four items together, and the round's pushes (near items onto the near queue or
+// work queue, items that stay onto the other far pile through the workgroup buffer) take one
+// reservation per queue for the wave.  The loop runs the same number of rounds on every lane.
 template <bool ASTAR>
 __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 size, u32* nearQ, u32* nearCnt, u32* nearMask,
                              u32* farOut, u32* farOutCnt, u32* lbFar, u32* lbFarN) {
-    const u32 n = r.g.n;
-    for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size; idx += gridDim.x * blockDim.x) {
-        const u32 gi = ldA(&far[idx]);
-        const u32 k = slotOf(r, gi);
-        const u32 d = distOf(ldA(&r.state[gi]));
-        u64 key = d;
-        bool ok = d != INF;
-        if constexpr (ASTAR) {
-            const u32 t = r.target[k];
-            key += heur(r, loadTarget(r, t), gi - k * n);
-            ok = ok && key <= (u64)distOf(r.state[k * n + t]);
+    const u32 stride = gridDim.x * blockDim.x;
+    const Outs o{nearQ, nearCnt, nearMask, farOut, farOutCnt, r.farMask, nullptr, r.async ? r.ring : nullptr,
+                 lbFar, lbFarN};
+    for (u32 i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 - threadIdx.x < size; i0 += 4 * stride) {
+        u32 gi[4], k[4], d[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            in[q] = i0 + q * stride < size;
+            gi[q] = in[q] ? ldA(&far[i0 + q * stride]) : 0u;
         }
-        ok = ok && key >= ldA(&r.bound[k]);
-        if (!ok) {
-            clearBit(r.farMask, gi);                 // stale or pruned
-        } else if (key < ldA(&r.newBound[k])) {
-            clearBit(r.farMask, gi);                 // leaves the far pile for the new bucket
-            if (testAndSetBit(nearMask, gi)) {
-                if (r.async) pushRing(r, gi); else pushQ(nearQ, nearCnt, gi);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            k[q] = slotOf(r, gi[q]);
+            d[q] = in[q] ? distOf(ldA(&r.state[gi[q]])) : INF;
+        }
+        unsigned long long key[4], nb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) key[q] = in[q] ? farKey<ASTAR>(r, gi[q], k[q], d[q]) : NOKEY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nb[q] = key[q] != NOKEY ? ldA(&r.newBound[k[q]]) : 0ull;
+        bool nearq[4], stay[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            nearq[q] = key[q] != NOKEY && key[q] < nb[q];      // leaves the far pile for the new bucket
+            stay[q] = key[q] != NOKEY && !nearq[q];            // stays (bit unchanged)
+            if (in[q] && !stay[q]) clearBit(r.farMask, gi[q]); // stale, pruned, or moving to near
+        }
+        u32 w[4];                                              // testAndSetBit, four at a time
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = nearq[q] ? ldA(&nearMask[gi[q] >> 5]) : 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const u32 bit = 1u << (gi[q] & 31u);
+            nearq[q] = nearq[q] && !(w[q] & bit);
+            w[q] = nearq[q] ? atomicOr(&nearMask[gi[q] >> 5], bit) : bit;
+        }
+        u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
+        u32 nn = 0, nf = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (nearq[q] && !(w[q] & (1u << (gi[q] & 31u)))) {
+#pragma unroll
+                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gi[q];   // constant indices: registers
+                ++nn;
             }
-        } else {
-            pushBuf(lbFar, lbFarN, LB_FAR, farOut, farOutCnt, gi);   // stays (bit unchanged)
+            if (stay[q]) {
+#pragma unroll
+                for (u32 j = 0; j < 4; ++j) if (j == nf) farV[j] = gi[q];
+                ++nf;
+            }
         }
+        pushGroup(r, o, nearV, nn, farV, nf);
     }
 }
 
@@ -1317,7 +1385,8 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
     __shared__ u32 s_owner;
     __shared__ Item s_item;
     __shared__ u32 waveTotals[17];
-    __shared__ u32 s_lbFar[LB_FAR], s_lbR[LB_R];   // workgroup push buffers (far pile, R)
+    __shared__ __align__(8) u32 s_lbFar[LB_FAR];   // workgroup push buffers (far pile, R); the far
+    __shared__ u32 s_lbR[LB_R];                     // buffer holds the slot minima in the far-min step
     __shared__ u32 s_lbFarN, s_lbRN, s_lbBase;
     LocalStats ls;
     Ctl* c = r.ctl;
@@ -1371,7 +1440,7 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break;
             case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
             case PH_SCAN_SCATTER: scanScatter(r, sel2(r.nearQ, in ^ 1u), waveTotals); break;
-            case PH_FARMIN: farMin<ASTAR>(r, sel2(r.farQ, fcur), size); break;
+            case PH_FARMIN: farMin<ASTAR>(r, sel2(r.farQ, fcur), size, reinterpret_cast<unsigned long long*>(s_lbFar)); break;
             case PH_PART:
                 farPartition<ASTAR>(r, sel2(r.farQ, fcur), size, sel2(r.nearQ, in), &c->cnt[C_NEAR0 + in], sel2(r.nearMask, in),
                                     sel2(r.farQ, fcur ^ 1u), &c->cnt[C_FAR0 + (fcur ^ 1u)], s_lbFar, &s_lbFarN);

@@ -70,6 +70,8 @@ CASES = [
     ("grid", 20, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=8"),
     ("rmat", 9, "delta", "map=2 grid=3 pers=1 K=3 kmax=4 astar=1 lm=4 sym=0"),
     ("grid", 20, "delta", "map=0 grid=2 K=2 kmax=4 astar=1 lm=8 yield=4 delta=8000"),
+    # more slots than the (shrunk) LDS slot minima hold: the far minimum's global path
+    ("grid", 16, "delta", "map=2 grid=3 pers=1 K=16 kmax=16 yield=5"),
     # asynchronous light phase (work queue per bucket)
     ("grid", 20, "delta", "async=1 grid=3 prof=1"),
     ("grid", 20, "delta", "async=1 map=0 grid=4 pers=1 K=3 kmax=4 yield=3"),
