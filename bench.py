@@ -81,9 +81,10 @@ def main():
     ap.add_argument("--serve", action="store_true", help="run the serving workload instead")
     ap.add_argument("--serve-rate", type=float, default=1500.0)
     ap.add_argument("--serve-requests", type=int, default=3000)
-    ap.add_argument("--grid-slots", type=int, default=512)
+    ap.add_argument("--grid-slots", type=int, default=0,
+                    help="A* slots of the grid engine (0 = as many as half the HBM holds, within the K*n limit)")
     ap.add_argument("--rmat-slots", type=int, default=24)
-    ap.add_argument("--epoch", type=int, default=48)
+    ap.add_argument("--epoch", type=int, default=96)
     ap.add_argument("--landmarks", type=int, default=8, help="ALT landmarks for the serving A* (0 = floor(Euclid))")
     ap.add_argument("--grid-mapping", type=int, default=0,
                     help="mapping of the serving grid engine (0 thread, 1 wave, 2 hybrid; -1 = the network's pick)")
@@ -468,6 +469,9 @@ def serve(args):
     rmat = P.rmat(22, 16, 22)
     log(f"[serve] graphs built in {time.time() - t0:.1f}s: grid n={grid.n} m={grid.m}, rmat n={rmat.n} m={rmat.m}")
     model = L.DeltaModel.load()
+    if args.grid_slots <= 0:   # memory-based admission: the slots half the HBM holds
+        from pathfinding_gpu_amd.serving import slots_for_memory
+        args.grid_slots = slots_for_memory(grid.n, int(P.device_info(0)["global_mem"]), fraction=0.5)
     pg = model.choose(grid, args.grid_slots)
     pr = model.choose(rmat, args.rmat_slots)
     if args.rmat_mapping >= 0:
