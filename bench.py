@@ -423,6 +423,12 @@ def bench_union(args):
         sv.multi_seed(seeds, o, copy_back=False)
         ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
         res["persistent_tuned"] = (float(np.mean(ts)), r)
+        # the same launch shape driven as one captured hipGraph of 64 steps, replayed until done
+        og = tuned.options(cus, delta=int(pick["delta"]))
+        og.drive, og.sync_every = P.Drive.Graph, 64
+        sv.multi_seed(seeds, og, copy_back=False)
+        ts, r = timed(lambda: sv.multi_seed(seeds, og, copy_back=False), max(2, args.steps // 2))
+        res["graph_tuned"] = (float(np.mean(ts)), r)
         log(f"[union] autotuned: {tuned}")
     reached, arcs = sv.last_reach()
     best = min(res, key=lambda k: res[k][0])
@@ -433,7 +439,8 @@ def bench_union(args):
            "config": {"workload": "512 grids, side U[64,512], seeds 1000+i", "n": u.n, "m": u.m, "best": best,
                       "delta": pick["delta"], "mapping": pick["mapping"], "auto_delta": auto},
            "variants_ms": {k: round(v[0], 3) for k, v in res.items()},
-           "graph_replays": res["graph_learned"][1].stats.launches}
+           "graph_replays": res["graph_learned"][1].stats.launches,
+           "graph_tuned_replays": res["graph_tuned"][1].stats.launches if "graph_tuned" in res else None}
     if not args.no_cpu:
         import oracle
         r = sv.multi_seed(seeds, P.GpuOptions(drive=P.Drive.Graph, sync_every=64, delta=pick["delta"],
