@@ -1,0 +1,50 @@
+"""bench.py's contract pieces that need no GPU: the one-GPU refusal, the traffic figures it reads
+from the committed counter summaries, and the shipped tuning database."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+
+from pathfinding_gpu_amd.autotune import Tuned
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["n_gpus"] == 2 and "one GPU" in out["error"]
+
+
+def test_traffic_figures_of_the_committed_profiles():
+    b = _bench()
+    for workload, path in b.TRAFFIC_PROFILES.items():
+        recs = [json.loads(line[len("traffic: "):]) for line in open(path) if line.startswith("traffic: ")]
+        assert len(recs) == 1, path
+        rec = recs[0]
+        # the figure is found for exactly the options it was measured with
+        assert b.measured_traffic(rec["options"], workload) == rec["fetch_bytes"] + rec["write_bytes"]
+        other = dict(rec["options"], grid_blocks=rec["options"]["grid_blocks"] + 1)
+        assert b.measured_traffic(other, workload) is None
+        # calibrated against the copy kernel: FETCH_SIZE counts half the copied bytes, WRITE_SIZE all
+        assert abs(rec["fetch_scale"] - 2.0) < 0.01 and abs(rec["write_scale"] - 1.0) < 0.01
+        assert rec["fetch_bytes"] > 0 and rec["write_bytes"] > 0
+
+
+def test_shipped_tuning_database():
+    db = json.load(open(os.path.join(ROOT, "pathfinding_gpu_amd", "data", "autotune.json")))
+    assert db and all(k.startswith("v3") for k in db)
+    for k, v in db.items():
+        t = Tuned(**v)
+        o = t.options(256, delta=int(k.rsplit(":", 1)[1]))
+        assert o.block_size in (128, 256, 512, 1024) and o.grid_blocks >= 1 and o.delta >= 1
