@@ -68,7 +68,9 @@ typedef struct {
     uint32_t profile_phases; /* 1 = record wall-clock time per step phase (pf_solver_phase_profile) */
     uint32_t async_backoff;  /* asynchronous light phase: longest idle wait of a wave between polls of the
                                 queue counters, in units of 64 clocks (0 = 32) */
-    uint32_t reserved[2];
+    uint32_t async_queue_slots; /* asynchronous light phase: work-queue slots (0 = 2*K*n; at least the seed
+                                   count); a bucket that outgrows the queue is rerun synchronously */
+    uint32_t reserved[1];
 } pf_options;
 
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */

@@ -437,7 +437,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     // re-pushes included); a bucket that needs more is rerun below with the synchronous phase
     const bool async = o.algo == PF_ALGO_DELTA && o.async_light != 0;
     if (async) {
-        const u64 cap = 2 * Kn;
+        const u64 cap = o.async_queue_slots ? std::max<u64>(o.async_queue_slots, spec.nseeds) : 2 * Kn;
         if (cap > s->ringCap) {
             if (s->d_ring) { hipFree(s->d_ring); s->bytes -= s->ringCap * 4; s->d_ring = nullptr; s->ringCap = 0; }
             if ((rc = s->dalloc(&s->d_ring, cap))) return rc;

@@ -75,6 +75,7 @@ class GpuOptions:
     async_light: bool = False   # Delta: each bucket's light phase as one asynchronous work-queue step
     profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
     async_backoff: int = 0      # async light phase: longest idle wait between queue polls (x64 clocks; 0 = 32)
+    async_queue_slots: int = 0  # async light phase: work-queue slots (0 = 2*K*n); overflow reruns synchronously
 
     def to_c(self) -> N.PfOptions:
         o = N.PfOptions()
@@ -86,6 +87,7 @@ class GpuOptions:
         o.async_light = 1 if self.async_light else 0
         o.profile_phases = 1 if self.profile_phases else 0
         o.async_backoff = self.async_backoff
+        o.async_queue_slots = self.async_queue_slots
         return o
 
 

@@ -409,3 +409,19 @@ def test_benchmark_configurations_full_size(orc, workload):
         d, pc = oracle_sssp(orc, g, s)
         assert np.array_equal(dist[k], d), (workload, k, int(s))
         assert np.array_equal(par[k], pc), (workload, k, int(s))
+
+
+def test_async_light_queue_overflow_reruns_synchronously(orc, grid100, sgrid100, rmat12, srmat12):
+    """A work queue too small for a bucket: the run is repeated with the synchronous light phase
+    by the engine, with the same results."""
+    for g, sv, s in ((grid100, sgrid100, 4321), (rmat12, srmat12, rmat12.max_degree_vertex())):
+        o = P.GpuOptions(algo=GA.DeltaStep, drive=DR.Persistent, async_light=True, async_queue_slots=64,
+                         delta=1 << 20, profile_phases=True)
+        r = sv.sssp(s, o)
+        assert_sssp_exact(orc, g, s, r, "async overflow")
+        assert "light_async" not in sv.phase_profile()         # the profile is the synchronous rerun's
+        srcs = [0, 17, 99, 1234]
+        ms = sv.multi_source(srcs, o, parents=True)
+        for k, src in enumerate(srcs):
+            d, pc = oracle_sssp(orc, g, src)
+            assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), src
