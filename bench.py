@@ -405,7 +405,7 @@ def bench_union(args):
     sv = P.GpuSolver(u)
     pick = L.DeltaModel.load().choose(gs[int(np.argmax(sides))], 1)
     auto = sv.auto_delta()
-    res = {}
+    res, opts = {}, {}
     for name, o in (("persistent_learned", P.GpuOptions(drive=P.Drive.Persistent, delta=pick["delta"],
                                                          mapping=P.Mapping(pick["mapping"]))),
                     ("graph_learned", P.GpuOptions(drive=P.Drive.Graph, sync_every=64, delta=pick["delta"],
@@ -414,6 +414,7 @@ def bench_union(args):
         sv.multi_seed(seeds, o, copy_back=False)
         ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
         res[name] = (float(np.mean(ts)), r)
+        opts[name] = o
     if not args.no_tune:   # launch shape (mapping, light phase, workgroups) at the learned Delta
         from pathfinding_gpu_amd.autotune import autotune
         cus = P.device_info(0)["cu_count"]
@@ -423,12 +424,14 @@ def bench_union(args):
         sv.multi_seed(seeds, o, copy_back=False)
         ts, r = timed(lambda: sv.multi_seed(seeds, o, copy_back=False), max(2, args.steps // 2))
         res["persistent_tuned"] = (float(np.mean(ts)), r)
+        opts["persistent_tuned"] = o
         # the same launch shape driven as one captured hipGraph of 64 steps, replayed until done
         og = tuned.options(cus, delta=int(pick["delta"]))
         og.drive, og.sync_every = P.Drive.Graph, 64
         sv.multi_seed(seeds, og, copy_back=False)
         ts, r = timed(lambda: sv.multi_seed(seeds, og, copy_back=False), max(2, args.steps // 2))
         res["graph_tuned"] = (float(np.mean(ts)), r)
+        opts["graph_tuned"] = og
         log(f"[union] autotuned: {tuned}")
     reached, arcs = sv.last_reach()
     best = min(res, key=lambda k: res[k][0])
@@ -443,8 +446,7 @@ def bench_union(args):
            "graph_tuned_replays": res["graph_tuned"][1].stats.launches if "graph_tuned" in res else None}
     if not args.no_cpu:
         import oracle
-        r = sv.multi_seed(seeds, P.GpuOptions(drive=P.Drive.Graph, sync_every=64, delta=pick["delta"],
-                                              mapping=P.Mapping(pick["mapping"])))
+        r = sv.multi_seed(seeds, opts[best])       # the timed variant's results, per instance
 
         def check(i):
             g = gs[i]
@@ -458,6 +460,7 @@ def bench_union(args):
         with ThreadPoolExecutor(16) as ex:
             ok = list(ex.map(check, range(len(gs))))
         out["bitexact_instances"] = f"{sum(ok)}/{len(ok)}"
+        out["checked_variant"] = best
         out["cpu_oracle_check_s"] = round(time.perf_counter() - a, 2)
     sv.close()
     print(json.dumps(out))
