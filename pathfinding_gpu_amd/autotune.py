@@ -11,8 +11,9 @@ Here they are searched per graph and batch size on the device itself:
   * grid_blocks       -- workgroups of the persistent launch, i.e. how the
                          frontier queue is chunked over the machine,
   * mapping           -- thread / wave / hybrid,
-  * async_light       -- Delta-stepping's light phase as one asynchronous work-queue
-                         step per bucket, or one step per light iteration.
+  * async_light       -- Delta-stepping's light phase as one step per light iteration (0),
+                         one asynchronous work-queue step per bucket (1), or the same
+                         over 64-element tiles relaxed in registers (2; symmetric graphs).
 
 Search: coordinate descent from the engine defaults, each trial the best of
 `reps` timed runs of the caller's representative workload; results are cached
@@ -35,7 +36,7 @@ CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pathfinding_gpu_amd", "
 # shape differs, then the launch shape.
 SPACE = {
     "mapping": (0, 1, 2),
-    "async_light": (0, 1),
+    "async_light": (0, 1, 2),
     "grid_mult": (0.5, 1, 2, 4),
     "block_size": (128, 256, 512, 1024),
     "wave_cutoff": (16, 32, 64, 128),
@@ -58,7 +59,7 @@ class Tuned:
         return GpuOptions(drive=Drive.Persistent, block_size=self.block_size, wave_cutoff=self.wave_cutoff,
                           block_cutoff=self.block_size * self.block_cutoff_mult,
                           grid_blocks=max(1, int(self.grid_mult * cus)), mapping=Mapping(self.mapping),
-                          async_light=bool(self.async_light), **kw)
+                          async_light=int(self.async_light), **kw)
 
 
 def signature(g: CSRGraph, batch: int, delta: int, kind: str = "") -> str:

@@ -44,7 +44,8 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 inline u32 divUp(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
-inline u64 maskWords(u64 bits) { return (bits + 31) / 32; }
+// u32 words of a bit mask, rounded up to whole 64-bit words (the tiled phase reads 64 bits at a time)
+inline u64 maskWords(u64 bits) { return (bits + 63) / 64 * 2; }
 
 using StepFn = void (*)(Run, u32);
 
@@ -86,6 +87,9 @@ struct pf_solver {
     u32* d_lm = nullptr;       // landmark distances [n][lmCount], internal order (ALT heuristic)
     u32* d_ring = nullptr;     // work queue of the asynchronous light phase
     u64 ringCap = 0;
+    u32* d_tileMask = nullptr; // tiled asynchronous light phase: queued bit per 64-element tile
+    u64 tileMaskWords = 0;
+    int symmetric = -1;        // every arc has a reverse arc of equal weight (-1: not checked yet)
     u32 lmCount = 0, lmSym = 0;
     std::vector<u32> perm, inv; // internal -> original, original -> internal (empty: identity)
     std::vector<u32> callerRow; // the caller's row_offsets (test hooks report in the caller's CSR)
@@ -341,6 +345,34 @@ Run baseRun(const pf_solver* s) {
     return run;
 }
 
+// Whether every arc (u, v, w) has a reverse arc (v, u, w) (checked once per solver, on the host).
+int isSymmetric(pf_solver* s, u32* out) {
+    if (s->symmetric < 0) {
+        std::vector<uint2> adj(s->m);
+        std::vector<u32> row(s->n + 1);
+        PF_HIP(hipMemcpy(adj.data(), s->d_adj, (size_t)s->m * 8, hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpy(row.data(), s->d_row, (size_t)(s->n + 1) * 4, hipMemcpyDeviceToHost));
+        std::vector<u64> key(s->m);
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (long long u = 0; u < (long long)s->n; ++u) {
+            for (u32 e = row[u]; e < row[u + 1]; ++e) key[e] = ((u64)adj[e].y << 32) | adj[e].x;
+            std::sort(key.begin() + row[u], key.begin() + row[u + 1]);
+        }
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
+        for (long long u = 0; u < (long long)s->n; ++u) {
+            for (u32 e = row[u]; e < row[u + 1] && !bad; ++e) {
+                const u32 v = adj[e].x;
+                const u64 want = ((u64)adj[e].y << 32) | (u64)u;
+                if (!std::binary_search(key.begin() + row[v], key.begin() + row[v + 1], want)) bad = 1;
+            }
+        }
+        s->symmetric = bad ? 0 : 1;
+    }
+    *out = (u32)s->symmetric;
+    return PF_OK;
+}
+
 int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
     int rc = setDev(s);
     if (rc) return rc;
@@ -447,6 +479,25 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         run.ringCap = cap;
         run.async = 1;
         startPhase = PH_LIGHT_ASYNC;
+        // tiled variant: SSSP on a graph whose arcs all have reverse arcs of equal weight
+        u32 sym = 0;
+        if (o.async_light == 2 && !spec.astar && o.atomic != PF_ATOMIC_CAS_RBA) {
+            if ((rc = isSymmetric(s, &sym))) return rc;
+        }
+        if (sym) {
+            const u64 words = maskWords(divUp(Kn, 64));
+            if (words > s->tileMaskWords) {
+                if (s->d_tileMask) { hipFree(s->d_tileMask); s->bytes -= s->tileMaskWords * 4; s->d_tileMask = nullptr; }
+                s->tileMaskWords = 0;
+                if ((rc = s->dalloc(&s->d_tileMask, words))) return rc;
+                s->tileMaskWords = words;
+            }
+            hipLaunchKernelGGL(fillKernel, dim3(std::min<u32>(divUp(words, 256), 4096u)), dim3(256), 0, s->s,
+                               s->d_tileMask, words, 0u);
+            PF_HIP(hipGetLastError());
+            run.tileMode = 1;
+            run.tileMask = s->d_tileMask;
+        }
     }
     run.profile = o.profile_phases ? 1u : 0u;
     run.napCap = o.async_backoff ? std::min<u32>(o.async_backoff, 4096u) : 32u;
@@ -943,7 +994,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_ring, s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_tileMask, s->d_ring, s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,
@@ -976,30 +1027,8 @@ int pf_solver_build_landmarks(pf_solver* s, uint32_t count, const uint32_t* ids,
     if (ids)
         for (u32 j = 0; j < count; ++j)
             if (ids[j] >= s->n) return fail(PF_ERR_INVALID, "landmark out of range");
-    // symmetric: every arc (u, v, w) has a reverse arc (v, u, w)
-    u32 sym = 1;
-    {
-        std::vector<uint2> adj(s->m);
-        std::vector<u32> row(s->n + 1);
-        PF_HIP(hipMemcpy(adj.data(), s->d_adj, (size_t)s->m * 8, hipMemcpyDeviceToHost));
-        PF_HIP(hipMemcpy(row.data(), s->d_row, (size_t)(s->n + 1) * 4, hipMemcpyDeviceToHost));
-        std::vector<u64> key(s->m);
-#pragma omp parallel for schedule(dynamic, 4096)
-        for (long long u = 0; u < (long long)s->n; ++u) {
-            for (u32 e = row[u]; e < row[u + 1]; ++e) key[e] = ((u64)adj[e].y << 32) | adj[e].x;
-            std::sort(key.begin() + row[u], key.begin() + row[u + 1]);
-        }
-        int bad = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
-        for (long long u = 0; u < (long long)s->n; ++u) {
-            for (u32 e = row[u]; e < row[u + 1] && !bad; ++e) {
-                const u32 v = adj[e].x;
-                const u64 want = ((u64)adj[e].y << 32) | (u64)u;
-                if (!std::binary_search(key.begin() + row[v], key.begin() + row[v + 1], want)) bad = 1;
-            }
-        }
-        sym = bad ? 0u : 1u;
-    }
+    u32 sym = 0;
+    if ((rc = isSymmetric(s, &sym))) return rc;
     if (s->d_lm) {
         hipFree(s->d_lm);
         s->bytes -= (u64)s->n * s->lmCount * 4;

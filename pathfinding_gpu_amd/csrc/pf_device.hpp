@@ -118,6 +118,8 @@ struct Run {
     u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
     u32 profile;                 // accumulate wall-clock time per phase (controller, PROF_* words)
     u32 napCap;                  // async light phase: longest idle back-off, s_sleep(1) rounds
+    u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs, SSSP)
+    u32* tileMask;               // tile mode: queued bit per tile [ceil(total/64) bits]
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -348,6 +350,7 @@ struct Outs {
     u32* ring;    // asynchronous light phase: near pushes go to the work queue
     u32* lbFar;   // workgroup buffer (LDS) of far pushes [LB_FAR], or null: push directly
     u32* lbFarN;  // its fill count (may exceed LB_FAR: the excess was pushed directly)
+    u32* tileMask;  // tile mode: near pushes queue the target's tile (once, by its queued bit), not the vertex
 };
 
 // Workgroup push buffers (LDS) for the far pile and R: a step's far and R pushes are
@@ -417,7 +420,13 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
             ls.pushes++;
         } else if (key < it.bnd) {
             if (testAndSetBit(o.nearMask, gv)) {
-                if (o.ring) pushRing(r, gv); else pushQ(o.nearQ, o.nearCnt, gv);
+                if (o.tileMask) {
+                    if (testAndSetBit(o.tileMask, gv >> 6)) pushRing(r, gv >> 6);
+                } else if (o.ring) {
+                    pushRing(r, gv);
+                } else {
+                    pushQ(o.nearQ, o.nearCnt, gv);
+                }
                 ls.pushes++;
             }
         } else {
@@ -578,9 +587,10 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
             stA(&o.flags[gv[q]], 1u);
             ls.pushes++;
         } else if (key[q] < x.bnd[q]) {
-            if (testAndSetBit(o.nearMask, gv[q])) {
+            if (testAndSetBit(o.nearMask, gv[q]) && (!o.tileMask || testAndSetBit(o.tileMask, gv[q] >> 6))) {
+                const u32 item = o.tileMask ? gv[q] >> 6 : gv[q];
 #pragma unroll
-                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gv[q];   // constant indices: registers
+                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = item;   // constant indices: registers
                 ++nn;
             }
         } else if (testAndSetBit(o.farMask, gv[q])) {
@@ -949,6 +959,222 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled asynchronous light phase (graphs whose every arc has a reverse arc of the same weight;
+// SSSP).  Work items are tiles of 64 consecutive element ids 64j .. 64j+63 (graphs with
+// coordinates are in Morton order, so one slot's tile is a compact patch).  A wave takes a tile:
+// it clears the tile's queued bit, then fetches and clears the tile's 64 near bits with one
+// 64-bit atomic, loads the 64 states, and relaxes the light arcs inside the tile in registers:
+// in each round every lane pulls from its neighbours in the tile that changed in the round
+// before (the reverse arc has the same weight), until no lane changes -- rounds at register and
+// cross-lane speed instead of one memory round trip chain per level.  Then each lane that was
+// queued or changed writes its state back (atomicMin), relaxes its light arcs leaving the tile
+// once, with its final distance (queueing the targets' tiles), and enters R if it has heavy
+// arcs; a lane whose new key reached the bucket bound goes to the far pile instead.  The result is
+// the same fixpoint: every distance and min-index parent equals the untiled phase's.
+// ---------------------------------------------------------------------------
+constexpr u32 TILE_ARCS = 4;   // light arcs per lane held in registers by the tiled phase (one Arcs4 round)
+
+template <int AM>
+__device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, u32* lbR, u32* lbRN, LocalStats& ls) {
+    const u32 n = r.g.n;
+    const u32 lane = laneId();
+    const u32 gi = j * 64u + lane;
+    const bool valid = gi < r.total;
+    // the tile's queued bit first, then its near bits (fetch-and-clear, ordered after the clear)
+    u32 lo = 0, hi = 0;
+    if (lane == 0) {
+        const u32 z = zeroAfter(clearBitRet(r.tileMask, j));
+        const unsigned long long a =
+            atomicAnd(reinterpret_cast<unsigned long long*>(r.nearMask[0]) + j + z, 0ull);
+        lo = (u32)a;
+        hi = (u32)(a >> 32);
+    }
+    lo = __shfl(lo, 0, 64);
+    hi = __shfl(hi, 0, 64);
+    const u64 queued = ((u64)hi << 32) | lo;
+    const u32 z = zeroAfter(lo | hi);                   // the state reads follow the fetch
+    const u32 k = valid ? slotOf(r, gi) : 0u;
+    const u32 u = valid ? gi - k * n : 0u;
+    const u64 st = valid ? ldA(&r.state[gi + z]) : STATE_INIT;
+    uint4 va = make_uint4(0u, 0u, 0u, 0u);
+    u32 r1 = 0;
+    if (valid) {
+        va = r.g.vrec[u];
+        r1 = r.g.vrec[u + 1].x;
+    }
+    const u64 bnd = valid ? ldA(&r.bound[k]) : 0ull;
+    const u32 deg = va.y - va.x;                        // light arcs [va.x, va.y)
+    const u32 maxDeg = waveMax(deg);
+    // the first TILE_ARCS light arcs in registers: weight, the neighbour's lane when it lies in this
+    // tile (else the lane itself), and the neighbour's caller id (the parent it would give)
+    uint2 ra[TILE_ARCS];
+    u32 src[TILE_ARCS], pid[TILE_ARCS];
+    bool inT[TILE_ARCS];
+#pragma unroll
+    for (u32 q = 0; q < TILE_ARCS; ++q) {
+        const bool ok = q < deg;
+        ra[q] = ok ? r.g.adj[va.x + q] : make_uint2(0u, 0u);
+        const u32 xe = k * n + ra[q].x;                 // the neighbour's element id (same slot)
+        inT[q] = ok && (xe >> 6) == j;
+        src[q] = inT[q] ? (xe & 63u) : lane;
+    }
+#pragma unroll
+    for (u32 q = 0; q < TILE_ARCS; ++q) pid[q] = __shfl(va.z, (int)src[q], 64);
+    bool active = valid && ((queued >> lane) & 1ull) && distOf(st) != INF && (u64)distOf(st) < bnd;
+    const bool wasQueued = active;
+    u64 cur = st;
+    while (__ballot(active)) {                          // wave-uniform rounds
+        const u32 dsend = active ? distOf(cur) : INF;   // what this lane offers its tile neighbours
+        u64 best = cur;
+#pragma unroll
+        for (u32 q = 0; q < TILE_ARCS; ++q) {
+            const u32 dx = __shfl(dsend, (int)src[q], 64);
+            if (inT[q] && dx != INF) {
+                const u64 nd = (u64)dx + ra[q].y;
+                if (nd < INF) {
+                    const u64 cand = pack((u32)nd, pid[q]);
+                    best = cand < best ? cand : best;
+                }
+            }
+        }
+        for (u32 q = TILE_ARCS; q < maxDeg; ++q) {      // arcs beyond the register set (same trip count on every lane)
+            const bool ok = q < deg;
+            const uint2 a = ok ? r.g.adj[va.x + q] : make_uint2(0u, 0u);
+            const u32 xe = k * n + a.x;
+            const bool in = ok && (xe >> 6) == j;
+            const u32 sl = in ? (xe & 63u) : lane;
+            const u32 dx = __shfl(dsend, (int)sl, 64);
+            const u32 px = __shfl(va.z, (int)sl, 64);
+            if (in && dx != INF) {
+                const u64 nd = (u64)dx + a.y;
+                if (nd < INF) {
+                    const u64 cand = pack((u32)nd, px);
+                    best = cand < best ? cand : best;
+                }
+            }
+        }
+        ls.edges += active ? deg : 0u;
+        const bool imp = best < cur;
+        cur = best;
+        active = imp && (u64)distOf(cur) < bnd;         // improved inside the bucket: propagates
+    }
+    // write back; lanes whose value is now the state's own expand their arcs leaving the tile
+    bool expand = wasQueued;
+    bool toFar = false;
+    if (valid && cur < st) {
+        ls.atomics++;
+        const u64 old = atomicMin((unsigned long long*)&r.state[gi], (unsigned long long)cur);
+        if (old > cur) {
+            ls.updates++;
+            if ((u64)distOf(cur) < bnd) expand = true;
+            else toFar = true;
+        }
+    }
+    if (toFar && testAndSetBit(r.farMask, gi)) {
+        ls.pushes++;
+        pushFar(o, gi);
+    }
+    if (expand && va.y < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
+    // arcs leaving the tile, four per round, with the final distance
+    Item it;
+    it.u = u; it.p = va.z; it.du = distOf(cur); it.k = k; it.ub = NOKEY; it.bnd = bnd;
+    const u32 olen = expand ? deg : 0u;
+    if (__ballot(olen != 0u)) {                         // the register arcs
+        Arcs4<false> x;
+#pragma unroll
+        for (u32 q = 0; q < 4; ++q) {
+            x.ok[q] = q < olen && !inT[q];              // inside the tile: already relaxed
+            x.a[q] = ra[q];
+            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = NOKEY; x.bnd[q] = bnd;
+        }
+        relaxCore4<false, AM>(r, x, o, ls);
+    }
+    const u32 omax = waveMax(olen);
+    for (u32 off = TILE_ARCS; off < omax; off += 4) {
+        Arcs4<false> x;
+#pragma unroll
+        for (u32 q = 0; q < 4; ++q) {
+            const u32 e = va.x + off + q;
+            x.ok[q] = off + q < olen;
+            x.a[q] = x.ok[q] ? r.g.adj[e] : make_uint2(0u, 0u);
+            x.ok[q] = x.ok[q] && ((k * n + x.a[q].x) >> 6) != j;
+            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = NOKEY; x.bnd[q] = bnd;
+        }
+        relaxCore4<false, AM>(r, x, o, ls);
+    }
+}
+
+template <int AM>
+__device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u32* lbFarN, u32* lbR,
+                                          u32* lbRN) {
+    Ctl* c = r.ctl;
+    unsigned long long* head = &r.bar[RING_HEAD];
+    unsigned long long* tail = &r.bar[RING_TAIL];
+    unsigned long long* done = &r.bar[RING_DONE];
+    const Outs o{nullptr, nullptr, r.nearMask[0], sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
+                 lbFar, lbFarN, r.tileMask};
+    constexpr u64 NOCLAIM = ~0ull;
+    const u32 lane = laneId();
+    const unsigned long long t0 = wall_clock64();
+    u64 my = NOCLAIM;       // the wave's claimed slot (the same on every lane)
+    u64 pending = 0;        // tiles finished and not yet added to the done counter
+    u32 nap = 0;
+    bool idle = false;
+    while (true) {                                      // wave-uniform
+        if (my == NOCLAIM) {                            // claim one slot for the wave
+            unsigned long long b = 0;
+            if (lane == 0) {
+                b = atomicAdd(head, 1ull);
+                ls.queueOps++;
+            }
+            my = ((u64)__shfl((u32)(b >> 32), 0, 64) << 32) | __shfl((u32)b, 0, 64);
+        }
+        u64 avail = ~0ull;
+        if (idle) {                                     // as in asyncLight: flush, then test for the end
+            u32 stop = 0;
+            unsigned long long t = 0;
+            if (lane == 0) {
+                const unsigned long long d = pending ? atomicAdd(done, (unsigned long long)pending) + pending : ldA(done);
+                const u32 zz = zeroAfter((u32)d);
+                t = ldA(tail + zz);
+                if (d >= t) stop = 1;
+                else if (wall_clock64() - t0 > r.spinTicks) { atomicOr(&c->error, (u32)E_BARRIER); stop = 1; }
+            }
+            pending = 0;
+            if (__shfl(stop, 0, 64)) break;
+            avail = ((u64)__shfl((u32)(t >> 32), 0, 64) << 32) | __shfl((u32)t, 0, 64);
+        }
+        u32 tile = RING_EMPTY;
+        if (my < r.ringCap && my < avail) {
+            u32 v = RING_EMPTY;
+            if (lane == 0) {
+                v = ldA(&r.ring[my]);
+                if (v != RING_EMPTY) stA(&r.ring[my], RING_EMPTY);
+            }
+            tile = __shfl(v, 0, 64);
+            if (tile != RING_EMPTY) my = NOCLAIM;
+        }
+        if (tile != RING_EMPTY) {
+            processTile<AM>(r, tile, o, lbR, lbRN, ls);
+            // the wave's pushes and its slot reset are performed before the tile counts as finished
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+            waitAll();
+            asm volatile("" ::: "memory");
+            pending += 1;
+            idle = false;
+            nap = 0;
+            continue;
+        }
+        if (idle) {
+            nap = 2u * nap + 1u < r.napCap ? 2u * nap + 1u : r.napCap;
+            for (u32 i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
+        }
+        idle = true;
+    }
+}
+
 // Naive synchronous relaxation, one thread per vertex (src/gpu_kernels.cuh:297-329).
 template <bool ASTAR, int AM>
 __device__ __forceinline__ void naiveSweep(const Run& r, u8* actIn, u8* actOut, u32* changed, LocalStats& ls) {
@@ -1067,7 +1293,7 @@ __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 s
                              u32* farOut, u32* farOutCnt, u32* lbFar, u32* lbFarN) {
     const u32 stride = gridDim.x * blockDim.x;
     const Outs o{nearQ, nearCnt, nearMask, farOut, farOutCnt, r.farMask, nullptr, r.async ? r.ring : nullptr,
-                 lbFar, lbFarN};
+                 lbFar, lbFarN, r.tileMode ? r.tileMask : nullptr};
     for (u32 i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 - threadIdx.x < size; i0 += 4 * stride) {
         u32 gi[4], k[4], d[4];
         bool in[4];
@@ -1096,19 +1322,24 @@ __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 s
         u32 w[4];                                              // testAndSetBit, four at a time
 #pragma unroll
         for (int q = 0; q < 4; ++q) w[q] = nearq[q] ? ldA(&nearMask[gi[q] >> 5]) : 0u;
+        bool fresh[4];                                         // newly queued by this lane
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const u32 bit = 1u << (gi[q] & 31u);
-            nearq[q] = nearq[q] && !(w[q] & bit);
-            w[q] = nearq[q] ? atomicOr(&nearMask[gi[q] >> 5], bit) : bit;
+            fresh[q] = nearq[q] && !(w[q] & bit) && (atomicOr(&nearMask[gi[q] >> 5], bit) & bit) == 0u;
+        }
+        if (r.tileMode) {   // tile mode: a newly queued vertex queues its tile unless the tile is queued
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fresh[q] = fresh[q] && testAndSetBit(r.tileMask, gi[q] >> 6);
         }
         u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
         u32 nn = 0, nf = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if (nearq[q] && !(w[q] & (1u << (gi[q] & 31u)))) {
+            if (fresh[q]) {
+                const u32 item = r.tileMode ? gi[q] >> 6 : gi[q];
 #pragma unroll
-                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = gi[q];   // constant indices: registers
+                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = item;   // constant indices: registers
                 ++nn;
             }
             if (stay[q]) {
@@ -1419,7 +1650,7 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             case PH_HEAVY: {
                 const Outs o{sel2(r.nearQ, in), &c->cnt[C_NEAR0 + in], sel2(r.nearMask, in),
                              sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr,
-                             s_lbFar, &s_lbFarN};
+                             s_lbFar, &s_lbFarN, r.async && r.tileMode ? r.tileMask : nullptr};
                 expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, s_lbR, &s_lbRN,
                                                &s_owner, &s_item, gather);
                 break;
@@ -1437,7 +1668,12 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
                                                &s_item, gather);
                 break;
             }
-            case PH_LIGHT_ASYNC: asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break;
+            case PH_LIGHT_ASYNC:
+                if constexpr (!ASTAR && AM != AM_CAS_RBA) {
+                    if (r.tileMode) { tileLight<AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break; }
+                }
+                asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN);
+                break;
             case PH_SCAN_TILES: scanTiles(r, waveTotals); break;
             case PH_SCAN_SCATTER: scanScatter(r, sel2(r.nearQ, in ^ 1u), waveTotals); break;
             case PH_FARMIN: farMin<ASTAR>(r, sel2(r.farQ, fcur), size, reinterpret_cast<unsigned long long*>(s_lbFar)); break;
@@ -1527,8 +1763,14 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         const u32 gi = seeds[i];
         r.state[gi] = pack(0u, NOPAR);
         if (mode == 0) {
-            if (r.async) r.ring[i] = gi;
-            else r.nearQ[0][i] = gi;
+            if (r.async && r.tileMode) {   // the seed's tile (several seeds may share one: taken once)
+                r.ring[i] = gi >> 6;
+                atomicOr(r.tileMask + ((gi >> 6) >> 5), 1u << ((gi >> 6) & 31u));
+            } else if (r.async) {
+                r.ring[i] = gi;
+            } else {
+                r.nearQ[0][i] = gi;
+            }
             atomicOr(r.nearMask[0] + (gi >> 5), 1u << (gi & 31u));
         } else {
             r.act[0][gi] = 1;

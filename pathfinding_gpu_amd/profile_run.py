@@ -61,7 +61,7 @@ def options_from(d: dict):
     return GpuOptions(drive=Drive.Persistent, delta=int(d["delta"]), mapping=Mapping(int(d.get("mapping", 2))),
                       block_size=int(d.get("block_size", 256)), wave_cutoff=int(d.get("wave_cutoff", 0)),
                       block_cutoff=int(d.get("block_cutoff", 0)), grid_blocks=int(d.get("grid_blocks", 0)),
-                      async_light=bool(d.get("async_light", 0)))
+                      async_light=int(d.get("async_light", 0)))
 
 
 def run(a):

@@ -72,7 +72,9 @@ class GpuOptions:
     block_cutoff: int = 0
     grid_blocks: int = 0
     max_steps: int = 0
-    async_light: bool = False   # Delta: each bucket's light phase as one asynchronous work-queue step
+    async_light: int = 0        # Delta: 1 = each bucket's light phase as one asynchronous work-queue step;
+                                # 2 = the same over 64-vertex tiles relaxed in registers (SSSP on symmetric
+                                # graphs; anything else runs as 1)
     profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
     async_backoff: int = 0      # async light phase: longest idle wait between queue polls (x64 clocks; 0 = 32)
     async_queue_slots: int = 0  # async light phase: work-queue slots (0 = 2*K*n); overflow reruns synchronously
@@ -84,7 +86,7 @@ class GpuOptions:
         o.delta, o.block_size, o.sync_every, o.drive = self.delta, self.block_size, self.sync_every, int(self.drive)
         o.wave_cutoff, o.block_cutoff, o.grid_blocks = self.wave_cutoff, self.block_cutoff, self.grid_blocks
         o.heuristic, o.max_steps = int(self.heuristic), self.max_steps
-        o.async_light = 1 if self.async_light else 0
+        o.async_light = int(self.async_light)
         o.profile_phases = 1 if self.profile_phases else 0
         o.async_backoff = self.async_backoff
         o.async_queue_slots = self.async_queue_slots

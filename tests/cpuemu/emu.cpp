@@ -5,7 +5,7 @@
 // hipMalloc leaves them, and checks dist and parent (A*: target distance and canonical path)
 // against the oracle.  Prints "ok: ..." or "FAIL: ...".
 //   emu g=grid|rmat size= algo=naive|frontier|scan|delta map= am= bs= grid= K= kmax= delta=
-//       pers= astar= lm= sym= src= yield= fill= seed=
+//       pers= astar= lm= sym= src= yield= fill= seed= symm= (R-MAT with reverse arcs) tile=
 #include "shim.h"
 #include "dev_emu_0.hpp"   // one copy of the device code per concurrent workgroup: the copies'
 #include "dev_emu_1.hpp"   // function-scope statics stand for each workgroup's LDS
@@ -44,7 +44,7 @@ template <class T> T* dalloc(size_t n, bool zero = false) {
     std::memset(p, zero ? 0 : g_fill, std::max<size_t>(n, 1) * sizeof(T));
     return p;
 }
-static u64 maskWords(u64 bits) { return (bits + 31) / 32; }
+static u64 maskWords(u64 bits) { return (bits + 63) / 64 * 2; }
 
 int main(int argc, char** argv) {
     std::map<std::string, std::string> A;
@@ -79,6 +79,10 @@ int main(int argc, char** argv) {
             const u64 mm = 16ull << size;
             src.resize(mm); dst.resize(mm); w.resize(mm);
             pf_gen_rmat(size, 16, 0.57, 0.19, 0.19, 100, 11, src.data(), dst.data(), w.data());
+            if (gi_("symm", 0)) {   // every arc with its reverse arc of equal weight
+                src.resize(2 * mm); dst.resize(2 * mm); w.resize(2 * mm);
+                for (u64 e = 0; e < mm; ++e) { src[mm + e] = dst[e]; dst[mm + e] = src[e]; w[mm + e] = w[e]; }
+            }
         }
         row.resize(n + 1); col.resize(src.size()); wt.resize(src.size());
         pf_build_csr(n, src.size(), src.data(), dst.data(), w.data(), row.data(), col.data(), wt.data(), &m);
@@ -169,6 +173,10 @@ int main(int argc, char** argv) {
         for (u64 i = 0; i < r.ringCap; ++i) r.ring[i] = RING_EMPTY;
         r.async = 1;
         startPhase = PH_LIGHT_ASYNC;
+        if (gi_("tile", 0)) {   // tiled variant (symmetric graphs: the grid generator's)
+            r.tileMode = 1;
+            r.tileMask = dalloc<u32>(maskWords((KnMax + 63) / 64), true);
+        }
     }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }
     r.profile = gi_("prof", 0) ? 1u : 0u;
@@ -199,8 +207,13 @@ int main(int argc, char** argv) {
         launches = 1;
     } else {
         while (r.ctl->phase != PH_DONE && launches < 200000) {
+            const u32 ph0 = r.ctl->phase;
             emuLaunch(grid, bs, grid <= 4, [&](u32 b) { stepB(grid <= 4 ? b : 0, r, 0u, am, map, astar); });
             ++launches;
+            if (gi_("trace", 0))   // host-driven steps: phase run, next phase, queue counts, bound of slot 0
+                std::printf("step %u: phase %u -> %u  near %u %u far %u %u R %u tail %llu bound0 %llu\n", launches, ph0,
+                            r.ctl->phase, r.ctl->cnt[0], r.ctl->cnt[1], r.ctl->cnt[2], r.ctl->cnt[3], r.ctl->cnt[4],
+                            (unsigned long long)r.bar[RING_TAIL], (unsigned long long)r.bound[0]);
         }
     }
     std::printf("phase %u steps %u error %u launches %u\n", r.ctl->phase, r.ctl->steps, r.ctl->error, launches);

@@ -34,5 +34,5 @@ def test_options_mapping():
     assert (o.block_size, o.wave_cutoff, o.block_cutoff, o.grid_blocks, int(o.mapping), o.delta) == \
         (1024, 32, 4096, 128, 1, 7)
     assert all(len(v) >= 3 for k, v in SPACE.items() if k != "async_light")
-    assert SPACE["async_light"] == (0, 1)
-    assert Tuned(async_light=1).options(256, delta=7).async_light is True
+    assert SPACE["async_light"] == (0, 1, 2)
+    assert Tuned(async_light=1).options(256, delta=7).async_light == 1

@@ -365,6 +365,60 @@ def test_async_light_full_size_grid(orc, grid1000):
         assert np.array_equal(r.dist, want.reshape(16, g.n))
 
 
+# ---------------------------------------------------------------- tiled asynchronous light phase
+@pytest.mark.parametrize("drive", [DR.Host, DR.Graph, DR.Persistent], ids=lambda a: a.name)
+@pytest.mark.parametrize("mapping", MAPS, ids=lambda a: a.name)
+def test_tiled_light_phase(orc, grid100, sgrid100, rmat12, srmat12, drive, mapping):
+    """async_light=2: each 64-element tile's light relaxations iterated in registers on the
+    symmetric grid (the R-MAT graph is directed: it runs the untiled queue); same dist and
+    parents as the oracle, and a different amount of work than the untiled queue on the grid."""
+    for g, sv, s in ((grid100, sgrid100, 4321), (rmat12, srmat12, rmat12.max_degree_vertex())):
+        for atomic in ATOMS:
+            o = P.GpuOptions(algo=GA.DeltaStep, mapping=mapping, atomic=atomic, drive=drive, async_light=2)
+            r = sv.sssp(s, o)
+            assert_sssp_exact(orc, g, s, r, f"tiled {drive.name}/{mapping.name}/{atomic.name}")
+            if g is grid100 and atomic != AM.CasRBA:
+                untiled = sv.sssp(s, P.GpuOptions(algo=GA.DeltaStep, mapping=mapping, atomic=atomic, drive=drive,
+                                                  async_light=1))
+                assert r.stats.edges_scanned != untiled.stats.edges_scanned
+
+
+@pytest.mark.parametrize("delta", [1, 37, 1000, 1 << 30])
+def test_tiled_light_delta_values_and_batches(orc, grid100, sgrid100, delta):
+    o = P.GpuOptions(algo=GA.DeltaStep, delta=delta, drive=DR.Persistent, async_light=2)
+    r = sgrid100.sssp(17, o)
+    assert_sssp_exact(orc, grid100, 17, r, f"tiled delta {delta}")
+    srcs = [0, 99, 5000, 9999, 1234, 42, 7777, 3]     # slot boundaries fall inside tiles (n % 64 != 0)
+    ms = sgrid100.multi_source(srcs, o, parents=True)
+    for k, s in enumerate(srcs):
+        d, pc = oracle_sssp(orc, grid100, s)
+        assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), (delta, s)
+
+
+def test_tiled_light_full_size_grid(orc, grid1000):
+    g = grid1000
+    with P.GpuSolver(g, max_batch=16) as sv:
+        auto = sv.auto_delta()
+        for mult in (1, 32):
+            for mp in (MP.Thread, MP.Hybrid):
+                r = sv.sssp(0, P.GpuOptions(drive=DR.Persistent, delta=auto * mult, mapping=mp, async_light=2))
+                assert_sssp_exact(orc, g, 0, r, f"tiled grid1000 delta {auto * mult} {mp.name}")
+        srcs = np.random.default_rng(7).choice(g.n, 16, replace=False)
+        r = sv.multi_source(srcs, P.GpuOptions(drive=DR.Persistent, delta=auto, async_light=2), parents=True)
+        dist, par = r.dist.reshape(16, g.n), r.parent.reshape(16, g.n)
+        for k, s in enumerate(srcs):
+            d, pc = oracle_sssp(orc, g, s)
+            assert np.array_equal(dist[k], d) and np.array_equal(par[k], pc), (k, int(s))
+
+
+def test_tiled_light_queue_overflow_reruns_synchronously(orc, grid100, sgrid100):
+    o = P.GpuOptions(algo=GA.DeltaStep, drive=DR.Persistent, async_light=2, async_queue_slots=8,
+                     delta=1 << 20, profile_phases=True)
+    r = sgrid100.sssp(4321, o)
+    assert_sssp_exact(orc, grid100, 4321, r, "tiled overflow")
+    assert "light_async" not in sgrid100.phase_profile()
+
+
 @pytest.mark.parametrize("async_light", [False, True])
 def test_phase_profile(orc, grid100, sgrid100, async_light):
     """The per-phase wall-clock profile counts every step of the run once, under the phase
