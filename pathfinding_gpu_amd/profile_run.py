@@ -120,18 +120,28 @@ def summarize(a):
     print("traffic: " + json.dumps(rec))
 
 
-def trace_summary(db: str):
+def trace_summary(db: str, calls: str = ""):
     """Per kernel and launch shape of a rocprofv3 --kernel-trace results database: calls, total,
-    average, min and max duration (the figures of rocprofv3 --stats)."""
+    average, min and max duration (the figures of rocprofv3 --stats).  With `calls` (a piece of a
+    kernel name), each launch of the matching kernels in dispatch order as well."""
     import collections
     con = sqlite3.connect(db)
     try:
-        rows = con.execute("select name, grid_x, workgroup_x, duration from kernels").fetchall()
+        rows = con.execute("select name, grid_x, workgroup_x, duration from kernels order by start").fetchall()
     finally:
         con.close()
     agg = collections.defaultdict(list)
     for name, gx, wx, d in rows:
         agg[(name.split("(")[0], gx // max(1, wx), wx)].append(d / 1000.0)
+    if calls:
+        print("| # | kernel | workgroups | wg size | us |")
+        print("|---|---|---|---|---|")
+        i = 0
+        for name, gx, wx, d in rows:
+            if calls in name:
+                i += 1
+                print(f"| {i} | {name.split('(')[0]} | {gx // max(1, wx)} | {wx} | {d / 1000.0:.2f} |")
+        print()
     tot = sum(sum(v) for v in agg.values())
     print("| kernel | workgroups | wg size | calls | total ms | avg us | min us | max us | % |")
     print("|---|---|---|---|---|---|---|---|---|")
@@ -149,9 +159,11 @@ def main():
                                                   "(delta, mapping, block_size, wave_cutoff, block_cutoff, grid_blocks)")
     ap.add_argument("--summarize", default="", help="directory holding FETCH_SIZE/ and WRITE_SIZE/ results")
     ap.add_argument("--trace-summary", default="", help="rocprofv3 --kernel-trace results database")
+    ap.add_argument("--calls", default="", help="with --trace-summary: list each launch of the kernels "
+                                                "whose name contains this text")
     a = ap.parse_args()
     if a.trace_summary:
-        trace_summary(a.trace_summary)
+        trace_summary(a.trace_summary, a.calls)
     elif a.summarize:
         summarize(a)
     else:
