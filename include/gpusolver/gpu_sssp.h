@@ -43,6 +43,8 @@ struct GpuOptions {
     u32 waveCutoff = 0;       // hybrid: degree >= -> one wave per vertex (0 = 64)
     u32 blockCutoff = 0;      // hybrid: degree >= -> whole workgroup (0 = blockSize)
     u32 gridBlocks = 0;       // workgroups per launch (0 = automatic)
+    u32 asyncLight = 0;       // Delta light phase: 0 one step per light iteration, 1 one work-queue step per
+                              // bucket, 2 the same over 64-vertex tiles (symmetric graphs; else as 1)
 };
 
 struct GpuStats {

@@ -35,6 +35,7 @@ pf_options toC(const GpuOptions& o) {
     c.wave_cutoff = o.waveCutoff;
     c.block_cutoff = o.blockCutoff;
     c.grid_blocks = o.gridBlocks;
+    c.async_light = o.asyncLight;
 #endif
     return c;
 }
