@@ -73,7 +73,8 @@ typedef struct {
                                 queue counters, in units of 64 clocks (0 = 32) */
     uint32_t async_queue_slots; /* asynchronous light phase: work-queue slots (0 = 2*K*n; at least the seed
                                    count); a bucket that outgrows the queue is rerun synchronously */
-    uint32_t reserved[1];
+    uint32_t async_tile_claim;  /* tiled light phase: work-queue slots a wave claims per update of the
+                                   queue head (0 = automatic: 1 for one query, 4 for batches) */
 } pf_options;
 
 /* GpuStats (gpu_sssp.h:55-65) + GpuRunResult.ms (gpu_sssp.h:67-74). */

@@ -19,7 +19,7 @@ class PfOptions(C.Structure):
         ("algo", C.c_int32), ("mapping", C.c_int32), ("atomic", C.c_int32), ("compaction", C.c_int32),
         ("delta", C.c_uint32), ("block_size", C.c_int32), ("sync_every", C.c_uint32), ("drive", C.c_int32),
         ("wave_cutoff", C.c_uint32), ("block_cutoff", C.c_uint32), ("grid_blocks", C.c_uint32),
-        ("heuristic", C.c_int32), ("max_steps", C.c_uint32), ("async_light", C.c_uint32), ("profile_phases", C.c_uint32), ("async_backoff", C.c_uint32), ("async_queue_slots", C.c_uint32), ("reserved", C.c_uint32),
+        ("heuristic", C.c_int32), ("max_steps", C.c_uint32), ("async_light", C.c_uint32), ("profile_phases", C.c_uint32), ("async_backoff", C.c_uint32), ("async_queue_slots", C.c_uint32), ("async_tile_claim", C.c_uint32),
     ]
 
 

@@ -120,6 +120,7 @@ struct Run {
     u32 napCap;                  // async light phase: longest idle back-off, s_sleep(1) rounds
     u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs, SSSP)
     u32* tileMask;               // tile mode: queued bit per tile [ceil(total/64) bits]
+    u32 tileClaim;               // tile mode: work-queue slots a wave claims per head update (>= 1)
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -1114,21 +1115,21 @@ __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls
     unsigned long long* done = &r.bar[RING_DONE];
     const Outs o{nullptr, nullptr, r.nearMask[0], sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
                  lbFar, lbFarN, r.tileMask};
-    constexpr u64 NOCLAIM = ~0ull;
     const u32 lane = laneId();
     const unsigned long long t0 = wall_clock64();
-    u64 my = NOCLAIM;       // the wave's claimed slot (the same on every lane)
+    u64 my = 0, myEnd = 0;  // the wave's claimed slots [my, myEnd), taken in order (the same on every lane)
     u64 pending = 0;        // tiles finished and not yet added to the done counter
     u32 nap = 0;
     bool idle = false;
     while (true) {                                      // wave-uniform
-        if (my == NOCLAIM) {                            // claim one slot for the wave
+        if (my == myEnd) {                              // claim the next tileClaim slots with one atomic
             unsigned long long b = 0;
             if (lane == 0) {
-                b = atomicAdd(head, 1ull);
+                b = atomicAdd(head, (unsigned long long)r.tileClaim);
                 ls.queueOps++;
             }
             my = ((u64)__shfl((u32)(b >> 32), 0, 64) << 32) | __shfl((u32)b, 0, 64);
+            myEnd = my + r.tileClaim;
         }
         u64 avail = ~0ull;
         if (idle) {                                     // as in asyncLight: flush, then test for the end
@@ -1153,7 +1154,7 @@ __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls
                 if (v != RING_EMPTY) stA(&r.ring[my], RING_EMPTY);
             }
             tile = __shfl(v, 0, 64);
-            if (tile != RING_EMPTY) my = NOCLAIM;
+            if (tile != RING_EMPTY) ++my;
         }
         if (tile != RING_EMPTY) {
             processTile<AM>(r, tile, o, lbR, lbRN, ls);

@@ -78,6 +78,7 @@ class GpuOptions:
     profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
     async_backoff: int = 0      # async light phase: longest idle wait between queue polls (x64 clocks; 0 = 32)
     async_queue_slots: int = 0  # async light phase: work-queue slots (0 = 2*K*n); overflow reruns synchronously
+    async_tile_claim: int = 0   # tiled light phase: queue slots a wave claims at once (0 = 1 for one query, 4 for batches)
 
     def to_c(self) -> N.PfOptions:
         o = N.PfOptions()
@@ -90,6 +91,7 @@ class GpuOptions:
         o.profile_phases = 1 if self.profile_phases else 0
         o.async_backoff = self.async_backoff
         o.async_queue_slots = self.async_queue_slots
+        o.async_tile_claim = self.async_tile_claim
         return o
 
 

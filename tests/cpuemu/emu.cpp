@@ -5,7 +5,7 @@
 // hipMalloc leaves them, and checks dist and parent (A*: target distance and canonical path)
 // against the oracle.  Prints "ok: ..." or "FAIL: ...".
 //   emu g=grid|rmat size= algo=naive|frontier|scan|delta map= am= bs= grid= K= kmax= delta=
-//       pers= astar= lm= sym= src= yield= fill= seed= symm= (R-MAT with reverse arcs) tile=
+//       pers= astar= lm= sym= src= yield= fill= seed= symm= (R-MAT with reverse arcs) tile= tclaim=
 #include "shim.h"
 #include "dev_emu_0.hpp"   // one copy of the device code per concurrent workgroup: the copies'
 #include "dev_emu_1.hpp"   // function-scope statics stand for each workgroup's LDS
@@ -175,6 +175,7 @@ int main(int argc, char** argv) {
         startPhase = PH_LIGHT_ASYNC;
         if (gi_("tile", 0)) {   // tiled variant (symmetric graphs: the grid generator's)
             r.tileMode = 1;
+            r.tileClaim = (u32)gi_("tclaim", 1);
             r.tileMask = dalloc<u32>(maskWords((KnMax + 63) / 64), true);
         }
     }

@@ -395,6 +395,19 @@ def test_tiled_light_delta_values_and_batches(orc, grid100, sgrid100, delta):
         assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), (delta, s)
 
 
+@pytest.mark.parametrize("claim", [1, 3, 8, 64])
+def test_tiled_light_claims(orc, grid100, sgrid100, claim):
+    """Waves claiming several queue slots at once (taken in order): same results."""
+    o = P.GpuOptions(algo=GA.DeltaStep, delta=1000, drive=DR.Persistent, async_light=2, async_tile_claim=claim)
+    srcs = [0, 99, 5000, 9999, 1234, 42, 7777, 3]
+    ms = sgrid100.multi_source(srcs, o, parents=True)
+    for k, s in enumerate(srcs):
+        d, pc = oracle_sssp(orc, grid100, s)
+        assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), (claim, s)
+    r = sgrid100.sssp(4321, o)
+    assert_sssp_exact(orc, grid100, 4321, r, f"tiled claim {claim}")
+
+
 def test_tiled_light_full_size_grid(orc, grid1000):
     g = grid1000
     with P.GpuSolver(g, max_batch=16) as sv:
