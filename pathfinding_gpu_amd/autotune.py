@@ -13,7 +13,9 @@ Here they are searched per graph and batch size on the device itself:
   * mapping           -- thread / wave / hybrid,
   * async_light       -- Delta-stepping's light phase as one step per light iteration (0),
                          one asynchronous work-queue step per bucket (1), or the same
-                         over 64-element tiles relaxed in registers (2; symmetric graphs).
+                         over 64-element tiles relaxed in registers (2; symmetric graphs),
+  * atomic            -- MinRBA (read before the atomicMin) or MinNoCheck (atomicMin
+                         directly: one round trip fewer per relaxation, more atomics).
 
 Search: coordinate descent from the engine defaults, each trial the best of
 `reps` timed runs of the caller's representative workload; results are cached
@@ -28,7 +30,7 @@ from dataclasses import asdict, dataclass
 from typing import Callable, Dict, Optional
 
 from .graph import CSRGraph
-from .solver import Drive, GpuOptions, Mapping
+from .solver import AtomicMode, Drive, GpuOptions, Mapping
 
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pathfinding_gpu_amd", "autotune.json")
 
@@ -37,6 +39,7 @@ CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pathfinding_gpu_amd", "
 SPACE = {
     "mapping": (0, 1, 2),
     "async_light": (0, 1, 2),
+    "atomic": (0, 2),
     "grid_mult": (0.5, 1, 2, 4),
     "block_size": (128, 256, 512, 1024),
     "wave_cutoff": (16, 32, 64, 128),
@@ -52,6 +55,7 @@ class Tuned:
     grid_mult: float = 2
     mapping: int = 2
     async_light: int = 0
+    atomic: int = 0
     ms: float = float("inf")
     trials: int = 0
 
@@ -59,7 +63,7 @@ class Tuned:
         return GpuOptions(drive=Drive.Persistent, block_size=self.block_size, wave_cutoff=self.wave_cutoff,
                           block_cutoff=self.block_size * self.block_cutoff_mult,
                           grid_blocks=max(1, int(self.grid_mult * cus)), mapping=Mapping(self.mapping),
-                          async_light=int(self.async_light), **kw)
+                          async_light=int(self.async_light), atomic=AtomicMode(self.atomic), **kw)
 
 
 def signature(g: CSRGraph, batch: int, delta: int, kind: str = "") -> str:

@@ -228,6 +228,12 @@ __device__ __forceinline__ bool testAndSetBit(u32* mask, u32 i) {
     if (ldA(wp) & bit) return false;
     return (atomicOr(wp, bit) & bit) == 0u;
 }
+// Sets bit i with one fetch-or and no read first (one round trip where the bit is usually clear);
+// true if this call set it.
+__device__ __forceinline__ bool setBitFresh(u32* mask, u32 i) {
+    const u32 bit = 1u << (i & 31u);
+    return (atomicOr(mask + (i >> 5), bit) & bit) == 0u;
+}
 __device__ __forceinline__ void clearBit(u32* mask, u32 i) { atomicAnd(mask + (i >> 5), ~(1u << (i & 31u))); }
 // Clears the bit and returns the old mask word (the atomic has been performed
 // at L2 once the value is back).
@@ -535,10 +541,11 @@ struct Arcs4 {
     Target t[ASTAR ? 4 : 1];
 };
 
-// Relaxes four arcs per lane, called by the whole wave: the read-before-atomic loads and the
-// atomicMins of the four arcs are each issued together before any result is used (the caller
-// issued the four arc loads together as well), so four arcs cost three dependent round trips
-// instead of twelve, and the pushes take one reservation per queue for the wave.
+// Relaxes four arcs per lane, called by the whole wave: the read-before-atomic loads, the
+// atomicMins and the queue-mark fetch-ors of the four arcs are each issued together before any
+// result is used (the caller issued the four arc loads together as well), so four arcs cost four
+// dependent round trips instead of sixteen, and the pushes take one reservation per queue for the
+// wave.
 template <bool ASTAR, int AM>
 __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, const Outs& o, LocalStats& ls) {
     static_assert(AM != AM_CAS_RBA, "the CAS loop relaxes one arc at a time");
@@ -578,23 +585,49 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
             old[q] = atomicMin((unsigned long long*)&r.state[gv[q]], (unsigned long long)nv[q]);
         }
     }
+    // the queue marks of the improved targets: the four fetch-ors issued together, no read first
+    // (most improvements find the mark clear); in tile mode a newly marked vertex marks its tile
+    bool nearC[4], farC[4];
+    u32 mw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool upd = go[q] && distOf(old[q]) > nd[q];
+        ls.updates += upd ? 1u : 0u;
+        if (upd && o.flags) {
+            stA(&o.flags[gv[q]], 1u);
+            ls.pushes++;
+        }
+        nearC[q] = upd && !o.flags && key[q] < x.bnd[q];
+        farC[q] = upd && !o.flags && !(key[q] < x.bnd[q]);
+        u32* m = nearC[q] ? o.nearMask : o.farMask;
+        mw[q] = (nearC[q] || farC[q]) ? atomicOr(&m[gv[q] >> 5], 1u << (gv[q] & 31u)) : ~0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool fresh = !(mw[q] & (1u << (gv[q] & 31u)));
+        nearC[q] = nearC[q] && fresh;
+        farC[q] = farC[q] && fresh;
+    }
+    if (o.tileMask) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const u32 t = gv[q] >> 6;
+            mw[q] = nearC[q] ? atomicOr(&o.tileMask[t >> 5], 1u << (t & 31u)) : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nearC[q] = nearC[q] && !(mw[q] & (1u << ((gv[q] >> 6) & 31u)));
+    }
     u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
     u32 nn = 0, nf = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (!(go[q] && distOf(old[q]) > nd[q])) continue;
-        ls.updates++;
-        if (o.flags) {
-            stA(&o.flags[gv[q]], 1u);
-            ls.pushes++;
-        } else if (key[q] < x.bnd[q]) {
-            if (testAndSetBit(o.nearMask, gv[q]) && (!o.tileMask || testAndSetBit(o.tileMask, gv[q] >> 6))) {
-                const u32 item = o.tileMask ? gv[q] >> 6 : gv[q];
+        if (nearC[q]) {
+            const u32 item = o.tileMask ? gv[q] >> 6 : gv[q];
 #pragma unroll
-                for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = item;   // constant indices: registers
-                ++nn;
-            }
-        } else if (testAndSetBit(o.farMask, gv[q])) {
+            for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = item;   // constant indices: registers
+            ++nn;
+        }
+        if (farC[q]) {
 #pragma unroll
             for (u32 j = 0; j < 4; ++j) if (j == nf) farV[j] = gv[q];
             ++nf;
@@ -894,6 +927,7 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
             Item it;
             it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
             it.ub = NOKEY; it.bnd = NOKEY;
+            bool markR = false;     // has heavy arcs: enters R (after the relaxations, off their chain)
             if (gi != RING_EMPTY) {
                 it.k = slotOf(r, gi);
                 it.u = gi - it.k * n;
@@ -910,7 +944,7 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
                 }
                 if (active) {
                     it.p = va.z;
-                    if (va.y < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &c->cnt[C_R], gi);
+                    markR = va.y < r1;
                     it.beg = va.x;
                     it.end = va.y;                       // light arcs
                     it.bnd = ldA(&r.bound[it.k]);
@@ -940,6 +974,7 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
             } else {
                 relaxGroups<ASTAR, AM>(r, it, busy && !big, o, ls);
             }
+            if (markR && setBitFresh(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &c->cnt[C_R], gi);
             // every lane of the wave is past its relaxations (the wave reconverges here), and the
             // wave's pushes and slot resets have been performed, before its items count as finished
             // (the empty asm statements keep the compiler from moving memory operations across the wait)
@@ -1285,7 +1320,7 @@ __device__ __forceinline__ void farMin(const Run& r, const u32* far, u32 size, u
 }
 
 // Far pile partition (src/gpu_kernels.cuh:413-436), four items per thread and round: the far
-// entries, their states, the new bounds, the mask clears, the near-mask checks and sets are each
+// entries, their states, the new bounds, the mask clears and the near-mask fetch-ors are each
 // issued for the four items together, and the round's pushes (near items onto the near queue or
 // work queue, items that stay onto the other far pile through the workgroup buffer) take one
 // reservation per queue for the wave.  The loop runs the same number of rounds on every lane.
@@ -1320,15 +1355,9 @@ __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 s
             stay[q] = key[q] != NOKEY && !nearq[q];            // stays (bit unchanged)
             if (in[q] && !stay[q]) clearBit(r.farMask, gi[q]); // stale, pruned, or moving to near
         }
-        u32 w[4];                                              // testAndSetBit, four at a time
+        bool fresh[4];                                         // newly queued by this lane: four fetch-ors together
 #pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = nearq[q] ? ldA(&nearMask[gi[q] >> 5]) : 0u;
-        bool fresh[4];                                         // newly queued by this lane
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const u32 bit = 1u << (gi[q] & 31u);
-            fresh[q] = nearq[q] && !(w[q] & bit) && (atomicOr(&nearMask[gi[q] >> 5], bit) & bit) == 0u;
-        }
+        for (int q = 0; q < 4; ++q) fresh[q] = nearq[q] && setBitFresh(nearMask, gi[q]);
         if (r.tileMode) {   // tile mode: a newly queued vertex queues its tile unless the tile is queued
 #pragma unroll
             for (int q = 0; q < 4; ++q) fresh[q] = fresh[q] && testAndSetBit(r.tileMask, gi[q] >> 6);
