@@ -210,7 +210,8 @@ def main():
     achieved = bytes_launch / (loop_ms * 1e-3) / 1e9
     timed_opts = {"delta": int(learnedK.delta), "mapping": int(learnedK.mapping), "block_size": int(learnedK.block_size),
                   "wave_cutoff": int(learnedK.wave_cutoff), "block_cutoff": int(learnedK.block_cutoff),
-                  "grid_blocks": int(learnedK.grid_blocks), "async_light": int(learnedK.async_light)}
+                  "grid_blocks": int(learnedK.grid_blocks), "async_light": int(learnedK.async_light),
+                  "atomic": int(learnedK.atomic)}
     traffic = measured_traffic(timed_opts, args.workload) if K == 64 else None
 
     out = {
@@ -246,7 +247,7 @@ def main():
         "learned_single_ms": round(ms_l1, 3),
         "learned_single_options": {"delta": int(learned1.delta), "mapping": int(learned1.mapping),
                                    "block_size": int(learned1.block_size), "grid_blocks": int(learned1.grid_blocks),
-                                   "async_light": int(learned1.async_light)},
+                                   "async_light": int(learned1.async_light), "atomic": int(learned1.atomic)},
         "learned_single_gteps": round(arcs1 / (ms_l1 * 1e-3) / 1e9, 5),
         "learned_single_speedup": round(ms_plain / ms_l1, 3),
         "step_stats": {"steps": r_step.stats.steps, "buckets": r_step.stats.buckets,

@@ -57,8 +57,9 @@ def workload(batch: int = BATCH, name: str = "grid1000"):
 
 
 def options_from(d: dict):
-    from .solver import Drive, GpuOptions, Mapping
+    from .solver import AtomicMode, Drive, GpuOptions, Mapping
     return GpuOptions(drive=Drive.Persistent, delta=int(d["delta"]), mapping=Mapping(int(d.get("mapping", 2))),
+                      atomic=AtomicMode(int(d.get("atomic", 0))),
                       block_size=int(d.get("block_size", 256)), wave_cutoff=int(d.get("wave_cutoff", 0)),
                       block_cutoff=int(d.get("block_cutoff", 0)), grid_blocks=int(d.get("grid_blocks", 0)),
                       async_light=int(d.get("async_light", 0)))
