@@ -714,14 +714,18 @@ __device__ __forceinline__ void expandFrontier(const Run& r, const u32* in, u32 
         Item it;
         it.u = 0; it.p = 0; it.du = INF; it.k = 0; it.beg = 0; it.end = 0;
         it.ub = NOKEY; it.bnd = NOKEY;
+        u32 gi = 0;
+        bool rMark = false;     // enters R: marked after the relaxations, off their chain of round trips
         if (idx < size) {
-            const u32 gi = ldA(&in[idx]);
+            gi = ldA(&in[idx]);
+            it.k = slotOf(r, gi);
+            it.u = gi - it.k * n;
+            const uint4 va = r.g.vrec[it.u];             // issued with the clear (independent of it)
+            const u32 r1 = r.g.vrec[it.u + 1].x;
             // The bit is cleared before the distance is read, and the read waits for the
             // clear: a relaxation that lowers dist[gi] after the read then finds the bit
             // clear and pushes gi again (it tests the bit only after its atomicMin).
             const u32 z = inMask ? zeroAfter(clearBitRet(inMask, gi)) : 0u;
-            it.k = slotOf(r, gi);
-            it.u = gi - it.k * n;
             it.du = distOf(ldA(&r.state[gi + z]));     // the latest distance: expanded with its final value
             bool active = it.du != INF;
             if constexpr (ASTAR) {
@@ -731,11 +735,10 @@ __device__ __forceinline__ void expandFrontier(const Run& r, const u32* in, u32 
                 if (active && (u64)it.du + heur(r, it.t, it.u) > it.ub) active = false;
             }
             if (active) {
-                const uint4 va = r.g.vrec[it.u];
-                const u32 r0 = va.x, le = va.y, r1 = r.g.vrec[it.u + 1].x;
+                const u32 r0 = va.x, le = va.y;
                 it.p = va.z;
                 // R only needs vertices that have heavy arcs (the heavy pass relaxes [le, r1))
-                if (markR && le < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
+                rMark = markR && le < r1;
                 if (sel == 0) { it.beg = r0; it.end = r1; }
                 else if (sel == 1) { it.beg = r0; it.end = le; }
                 else { it.beg = le; it.end = r1; }
@@ -844,6 +847,7 @@ __device__ __forceinline__ void expandFrontier(const Run& r, const u32* in, u32 
             }
             __syncthreads();   // the staging area is rewritten by the next chunk
         }
+        if (rMark && setBitFresh(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
     }
 }
 
@@ -1107,11 +1111,10 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
             else toFar = true;
         }
     }
-    if (toFar && testAndSetBit(r.farMask, gi)) {
+    if (toFar && setBitFresh(r.farMask, gi)) {
         ls.pushes++;
         pushFar(o, gi);
     }
-    if (expand && va.y < r1 && testAndSetBit(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
     // arcs leaving the tile, four per round, with the final distance
     Item it;
     it.u = u; it.p = va.z; it.du = distOf(cur); it.k = k; it.ub = NOKEY; it.bnd = bnd;
@@ -1139,6 +1142,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
         }
         relaxCore4<false, AM>(r, x, o, ls);
     }
+    if (expand && va.y < r1 && setBitFresh(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
 }
 
 template <int AM>
