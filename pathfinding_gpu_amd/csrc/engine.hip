@@ -87,8 +87,6 @@ struct pf_solver {
     u32* d_lm = nullptr;       // landmark distances [n][lmCount], internal order (ALT heuristic)
     u32* d_ring = nullptr;     // work queue of the asynchronous light phase
     u64 ringCap = 0;
-    u32* d_tileMask = nullptr; // tiled asynchronous light phase: queued bit per 64-element tile
-    u64 tileMaskWords = 0;
     int symmetric = -1;        // every arc has a reverse arc of equal weight (-1: not checked yet)
     u32 lmCount = 0, lmSym = 0;
     std::vector<u32> perm, inv; // internal -> original, original -> internal (empty: identity)
@@ -485,18 +483,7 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
             if ((rc = isSymmetric(s, &sym))) return rc;
         }
         if (sym) {
-            const u64 words = maskWords(divUp(Kn, 64));
-            if (words > s->tileMaskWords) {
-                if (s->d_tileMask) { hipFree(s->d_tileMask); s->bytes -= s->tileMaskWords * 4; s->d_tileMask = nullptr; }
-                s->tileMaskWords = 0;
-                if ((rc = s->dalloc(&s->d_tileMask, words))) return rc;
-                s->tileMaskWords = words;
-            }
-            hipLaunchKernelGGL(fillKernel, dim3(std::min<u32>(divUp(words, 256), 4096u)), dim3(256), 0, s->s,
-                               s->d_tileMask, words, 0u);
-            PF_HIP(hipGetLastError());
             run.tileMode = 1;
-            run.tileMask = s->d_tileMask;
             // one slot per claim for a single query (latency-bound: no tile may wait behind a busy
             // wave); batches queue many more tiles, and one head counter serializes the claims
             run.tileClaim = o.async_tile_claim ? std::min<u32>(o.async_tile_claim, 64u) : (spec.k > 1 ? 4u : 1u);
@@ -997,7 +984,7 @@ void pf_solver_destroy(pf_solver* s) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
-    void* ptrs[] = {s->d_tileMask, s->d_ring, s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
+    void* ptrs[] = {s->d_ring, s->d_lm, s->d_inv, s->d_row, s->d_adj, s->d_vrec, s->d_xyi, s->d_xyd, s->d_state, s->d_near[0], s->d_near[1],
                     s->d_nearMask[0], s->d_nearMask[1], s->d_far[0], s->d_far[1], s->d_farMask, s->d_r, s->d_rMask,
                     s->d_flags, s->d_tileSums, s->d_act[0], s->d_act[1], s->d_bound, s->d_newBound, s->d_minKey,
                     s->d_target, s->d_slotDone, s->d_src, s->d_ctl, s->d_bar, s->d_stats, s->d_reach, s->d_seeds, s->d_paths,

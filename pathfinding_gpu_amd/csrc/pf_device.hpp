@@ -118,8 +118,9 @@ struct Run {
     u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
     u32 profile;                 // accumulate wall-clock time per phase (controller, PROF_* words)
     u32 napCap;                  // async light phase: longest idle back-off, s_sleep(1) rounds
-    u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs, SSSP)
-    u32* tileMask;               // tile mode: queued bit per tile [ceil(total/64) bits]
+    u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs, SSSP);
+                                 // a tile's near bits are one 64-bit word of nearMask[0], and a tile is queued
+                                 // exactly while that word is non-zero (the push that makes it non-zero queues it)
     u32 tileClaim;               // tile mode: work-queue slots a wave claims per head update (>= 1)
 };
 
@@ -357,7 +358,7 @@ struct Outs {
     u32* ring;    // asynchronous light phase: near pushes go to the work queue
     u32* lbFar;   // workgroup buffer (LDS) of far pushes [LB_FAR], or null: push directly
     u32* lbFarN;  // its fill count (may exceed LB_FAR: the excess was pushed directly)
-    u32* tileMask;  // tile mode: near pushes queue the target's tile (once, by its queued bit), not the vertex
+    bool tiles;   // tile mode: a near mark that makes its tile's 64-bit word non-zero queues the tile
 };
 
 // Workgroup push buffers (LDS) for the far pile and R: a step's far and R pushes are
@@ -426,10 +427,12 @@ __device__ __forceinline__ void relaxEdge(const Run& r, const Item& it, uint2 a,
             stA(&o.flags[gv], 1u);
             ls.pushes++;
         } else if (key < it.bnd) {
-            if (testAndSetBit(o.nearMask, gv)) {
-                if (o.tileMask) {
-                    if (testAndSetBit(o.tileMask, gv >> 6)) pushRing(r, gv >> 6);
-                } else if (o.ring) {
+            if (o.tiles) {
+                const unsigned long long b = 1ull << (gv & 63u);
+                const unsigned long long was = atomicOr(reinterpret_cast<unsigned long long*>(o.nearMask) + (gv >> 6), b);
+                if (was == 0ull) { pushRing(r, gv >> 6); ls.pushes++; }
+            } else if (testAndSetBit(o.nearMask, gv)) {
+                if (o.ring) {
                     pushRing(r, gv);
                 } else {
                     pushQ(o.nearQ, o.nearCnt, gv);
@@ -586,9 +589,10 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
         }
     }
     // the queue marks of the improved targets: the four fetch-ors issued together, no read first
-    // (most improvements find the mark clear); in tile mode a newly marked vertex marks its tile
+    // (most improvements find the mark clear).  Tile mode: the near mark is a bit of the tile's
+    // 64-bit word, and the mark that finds the word zero queues the tile.
     bool nearC[4], farC[4];
-    u32 mw[4];
+    unsigned long long mw[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool upd = go[q] && distOf(old[q]) > nd[q];
@@ -599,30 +603,27 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
         }
         nearC[q] = upd && !o.flags && key[q] < x.bnd[q];
         farC[q] = upd && !o.flags && !(key[q] < x.bnd[q]);
-        u32* m = nearC[q] ? o.nearMask : o.farMask;
-        mw[q] = (nearC[q] || farC[q]) ? atomicOr(&m[gv[q] >> 5], 1u << (gv[q] & 31u)) : ~0u;
+        if (nearC[q] && o.tiles) {
+            mw[q] = atomicOr(reinterpret_cast<unsigned long long*>(o.nearMask) + (gv[q] >> 6), 1ull << (gv[q] & 63u));
+        } else if (nearC[q] || farC[q]) {
+            u32* m = nearC[q] ? o.nearMask : o.farMask;
+            mw[q] = atomicOr(&m[gv[q] >> 5], 1u << (gv[q] & 31u));
+        } else {
+            mw[q] = ~0ull;
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const bool fresh = !(mw[q] & (1u << (gv[q] & 31u)));
-        nearC[q] = nearC[q] && fresh;
+        const bool fresh = !(mw[q] & (1ull << (gv[q] & 31u)));
+        nearC[q] = nearC[q] && (o.tiles ? mw[q] == 0ull : fresh);
         farC[q] = farC[q] && fresh;
-    }
-    if (o.tileMask) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const u32 t = gv[q] >> 6;
-            mw[q] = nearC[q] ? atomicOr(&o.tileMask[t >> 5], 1u << (t & 31u)) : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) nearC[q] = nearC[q] && !(mw[q] & (1u << ((gv[q] >> 6) & 31u)));
     }
     u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
     u32 nn = 0, nf = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (nearC[q]) {
-            const u32 item = o.tileMask ? gv[q] >> 6 : gv[q];
+            const u32 item = o.tiles ? gv[q] >> 6 : gv[q];
 #pragma unroll
             for (u32 j = 0; j < 4; ++j) if (j == nn) nearV[j] = item;   // constant indices: registers
             ++nn;
@@ -1002,9 +1003,11 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
 // ---------------------------------------------------------------------------
 // Tiled asynchronous light phase (graphs whose every arc has a reverse arc of the same weight;
 // SSSP).  Work items are tiles of 64 consecutive element ids 64j .. 64j+63 (graphs with
-// coordinates are in Morton order, so one slot's tile is a compact patch).  A wave takes a tile:
-// it clears the tile's queued bit, then fetches and clears the tile's 64 near bits with one
-// 64-bit atomic, loads the 64 states, and relaxes the light arcs inside the tile in registers:
+// coordinates are in Morton order, so one slot's tile is a compact patch).  A tile's 64 near bits
+// are one 64-bit word, and the tile is in the queue exactly while its word is non-zero: the near
+// mark that finds the word zero queues the tile.  A wave takes a tile: it fetches and clears the
+// word with one 64-bit atomic (a mark after that finds it zero and queues the tile again), loads
+// the 64 states, and relaxes the light arcs inside the tile in registers:
 // in each round every lane pulls from its neighbours in the tile that changed in the round
 // before (the reverse arc has the same weight), until no lane changes -- rounds at register and
 // cross-lane speed instead of one memory round trip chain per level.  Then each lane that was
@@ -1021,12 +1024,10 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     const u32 lane = laneId();
     const u32 gi = j * 64u + lane;
     const bool valid = gi < r.total;
-    // the tile's queued bit first, then its near bits (fetch-and-clear, ordered after the clear)
+    // the tile's near bits, fetched and cleared at once
     u32 lo = 0, hi = 0;
     if (lane == 0) {
-        const u32 z = zeroAfter(clearBitRet(r.tileMask, j));
-        const unsigned long long a =
-            atomicAnd(reinterpret_cast<unsigned long long*>(r.nearMask[0]) + j + z, 0ull);
+        const unsigned long long a = atomicAnd(reinterpret_cast<unsigned long long*>(r.nearMask[0]) + j, 0ull);
         lo = (u32)a;
         hi = (u32)(a >> 32);
     }
@@ -1153,7 +1154,7 @@ __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls
     unsigned long long* tail = &r.bar[RING_TAIL];
     unsigned long long* done = &r.bar[RING_DONE];
     const Outs o{nullptr, nullptr, r.nearMask[0], sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.ring,
-                 lbFar, lbFarN, r.tileMask};
+                 lbFar, lbFarN, true};
     const u32 lane = laneId();
     const unsigned long long t0 = wall_clock64();
     u64 my = 0, myEnd = 0;  // the wave's claimed slots [my, myEnd), taken in order (the same on every lane)
@@ -1333,7 +1334,7 @@ __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 s
                              u32* farOut, u32* farOutCnt, u32* lbFar, u32* lbFarN) {
     const u32 stride = gridDim.x * blockDim.x;
     const Outs o{nearQ, nearCnt, nearMask, farOut, farOutCnt, r.farMask, nullptr, r.async ? r.ring : nullptr,
-                 lbFar, lbFarN, r.tileMode ? r.tileMask : nullptr};
+                 lbFar, lbFarN, r.tileMode != 0};
     for (u32 i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 - threadIdx.x < size; i0 += 4 * stride) {
         u32 gi[4], k[4], d[4];
         bool in[4];
@@ -1359,12 +1360,16 @@ __device__ __forceinline__ void farPartition(const Run& r, const u32* far, u32 s
             stay[q] = key[q] != NOKEY && !nearq[q];            // stays (bit unchanged)
             if (in[q] && !stay[q]) clearBit(r.farMask, gi[q]); // stale, pruned, or moving to near
         }
-        bool fresh[4];                                         // newly queued by this lane: four fetch-ors together
+        // newly queued by this lane, four fetch-ors together (tile mode: the mark that finds its
+        // tile's word zero queues the tile)
+        bool fresh[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fresh[q] = nearq[q] && setBitFresh(nearMask, gi[q]);
-        if (r.tileMode) {   // tile mode: a newly queued vertex queues its tile unless the tile is queued
-#pragma unroll
-            for (int q = 0; q < 4; ++q) fresh[q] = fresh[q] && testAndSetBit(r.tileMask, gi[q] >> 6);
+        for (int q = 0; q < 4; ++q) {
+            if (r.tileMode)
+                fresh[q] = nearq[q] && atomicOr(reinterpret_cast<unsigned long long*>(nearMask) + (gi[q] >> 6),
+                                                1ull << (gi[q] & 63u)) == 0ull;
+            else
+                fresh[q] = nearq[q] && setBitFresh(nearMask, gi[q]);
         }
         u32 nearV[4] = {0u, 0u, 0u, 0u}, farV[4] = {0u, 0u, 0u, 0u};
         u32 nn = 0, nf = 0;
@@ -1684,7 +1689,7 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
             case PH_HEAVY: {
                 const Outs o{sel2(r.nearQ, in), &c->cnt[C_NEAR0 + in], sel2(r.nearMask, in),
                              sel2(r.farQ, fcur), &c->cnt[C_FAR0 + fcur], r.farMask, nullptr, r.async ? r.ring : nullptr,
-                             s_lbFar, &s_lbFarN, r.async && r.tileMode ? r.tileMask : nullptr};
+                             s_lbFar, &s_lbFarN, r.async && r.tileMode};
                 expandFrontier<ASTAR, AM, MAP>(r, r.rQ, size, r.rMask, 2, false, true, o, ls, s_lbR, &s_lbRN,
                                                &s_owner, &s_item, gather);
                 break;
@@ -1797,9 +1802,8 @@ __global__ void seedKernel(Run r, const u32* seeds, u32 nseeds, u32 mode, u32 st
         const u32 gi = seeds[i];
         r.state[gi] = pack(0u, NOPAR);
         if (mode == 0) {
-            if (r.async && r.tileMode) {   // the seed's tile (several seeds may share one: taken once)
+            if (r.async && r.tileMode) {   // the seed's tile (a tile queued twice: the second take finds no bits)
                 r.ring[i] = gi >> 6;
-                atomicOr(r.tileMask + ((gi >> 6) >> 5), 1u << ((gi >> 6) & 31u));
             } else if (r.async) {
                 r.ring[i] = gi;
             } else {

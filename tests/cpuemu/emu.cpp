@@ -176,7 +176,6 @@ int main(int argc, char** argv) {
         if (gi_("tile", 0)) {   // tiled variant (symmetric graphs: the grid generator's)
             r.tileMode = 1;
             r.tileClaim = (u32)gi_("tclaim", 1);
-            r.tileMask = dalloc<u32>(maskWords((KnMax + 63) / 64), true);
         }
     }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }

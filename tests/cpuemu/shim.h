@@ -158,6 +158,10 @@ inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { emuMaybeYield(); return __a
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { emuMaybeYield(); return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline uint32_t atomicOr(uint32_t* p, uint32_t v) { emuMaybeYield(); return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline uint32_t atomicAnd(uint32_t* p, uint32_t v) { emuMaybeYield(); return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+    emuMaybeYield();
+    return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
 inline unsigned long long atomicAnd(unsigned long long* p, unsigned long long v) {
     emuMaybeYield();
     return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST);
