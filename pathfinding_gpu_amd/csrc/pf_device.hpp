@@ -1024,20 +1024,12 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     const u32 lane = laneId();
     const u32 gi = j * 64u + lane;
     const bool valid = gi < r.total;
-    // the tile's near bits, fetched and cleared at once
-    u32 lo = 0, hi = 0;
-    if (lane == 0) {
-        const unsigned long long a = atomicAnd(reinterpret_cast<unsigned long long*>(r.nearMask[0]) + j, 0ull);
-        lo = (u32)a;
-        hi = (u32)(a >> 32);
-    }
-    lo = __shfl(lo, 0, 64);
-    hi = __shfl(hi, 0, 64);
-    const u64 queued = ((u64)hi << 32) | lo;
-    const u32 z = zeroAfter(lo | hi);                   // the state reads follow the fetch
+    // the tile's near bits, fetched and cleared at once; the graph data loads below are issued
+    // while the fetch is in flight, and the state reads follow the fetch
+    unsigned long long fetched = 0;
+    if (lane == 0) fetched = atomicAnd(reinterpret_cast<unsigned long long*>(r.nearMask[0]) + j, 0ull);
     const u32 k = valid ? slotOf(r, gi) : 0u;
     const u32 u = valid ? gi - k * n : 0u;
-    const u64 st = valid ? ldA(&r.state[gi + z]) : STATE_INIT;
     uint4 va = make_uint4(0u, 0u, 0u, 0u);
     u32 r1 = 0;
     if (valid) {
@@ -1062,6 +1054,11 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     }
 #pragma unroll
     for (u32 q = 0; q < TILE_ARCS; ++q) pid[q] = __shfl(va.z, (int)src[q], 64);
+    const u32 lo = __shfl((u32)fetched, 0, 64);
+    const u32 hi = __shfl((u32)(fetched >> 32), 0, 64);
+    const u64 queued = ((u64)hi << 32) | lo;
+    const u32 z = zeroAfter(lo | hi);
+    const u64 st = valid ? ldA(&r.state[gi + z]) : STATE_INIT;
     bool active = valid && ((queued >> lane) & 1ull) && distOf(st) != INF && (u64)distOf(st) < bnd;
     const bool wasQueued = active;
     u64 cur = st;
