@@ -477,9 +477,9 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
         run.ringCap = cap;
         run.async = 1;
         startPhase = PH_LIGHT_ASYNC;
-        // tiled variant: SSSP on a graph whose arcs all have reverse arcs of equal weight
+        // tiled variant: a graph whose arcs all have reverse arcs of equal weight
         u32 sym = 0;
-        if (o.async_light == 2 && !spec.astar && o.atomic != PF_ATOMIC_CAS_RBA) {
+        if (o.async_light == 2 && o.atomic != PF_ATOMIC_CAS_RBA) {
             if ((rc = isSymmetric(s, &sym))) return rc;
         }
         if (sym) {

@@ -1018,7 +1018,7 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
 // ---------------------------------------------------------------------------
 constexpr u32 TILE_ARCS = 4;   // light arcs per lane held in registers by the tiled phase (one Arcs4 round)
 
-template <int AM>
+template <bool ASTAR, int AM>
 __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, u32* lbR, u32* lbRN, LocalStats& ls) {
     const u32 n = r.g.n;
     const u32 lane = laneId();
@@ -1037,6 +1037,19 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
         r1 = r.g.vrec[u + 1].x;
     }
     const u64 bnd = valid ? ldA(&r.bound[k]) : 0ull;
+    // A*: keys are distance + h(own vertex); a key above the slot's dist[target] is pruned (a
+    // stale, larger dist[target] only prunes less)
+    Target tg{};
+    u64 ub = NOKEY;
+    u32 h = 0;
+    if constexpr (ASTAR) {
+        if (valid) {
+            const u32 t = r.target[k];
+            tg = loadTarget(r, t);
+            ub = distOf(r.state[k * n + t]);
+            h = heur(r, tg, u);
+        }
+    }
     const u32 deg = va.y - va.x;                        // light arcs [va.x, va.y)
     const u32 maxDeg = waveMax(deg);
     // the first TILE_ARCS light arcs in registers: weight, the neighbour's lane when it lies in this
@@ -1059,7 +1072,8 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     const u64 queued = ((u64)hi << 32) | lo;
     const u32 z = zeroAfter(lo | hi);
     const u64 st = valid ? ldA(&r.state[gi + z]) : STATE_INIT;
-    bool active = valid && ((queued >> lane) & 1ull) && distOf(st) != INF && (u64)distOf(st) < bnd;
+    bool active = valid && ((queued >> lane) & 1ull) && distOf(st) != INF && (u64)distOf(st) + h < bnd &&
+                  (u64)distOf(st) + h <= ub;
     const bool wasQueued = active;
     u64 cur = st;
     while (__ballot(active)) {                          // wave-uniform rounds
@@ -1070,7 +1084,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
             const u32 dx = __shfl(dsend, (int)src[q], 64);
             if (inT[q] && dx != INF) {
                 const u64 nd = (u64)dx + ra[q].y;
-                if (nd < INF) {
+                if (nd < INF && nd + h <= ub) {
                     const u64 cand = pack((u32)nd, pid[q]);
                     best = cand < best ? cand : best;
                 }
@@ -1086,7 +1100,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
             const u32 px = __shfl(va.z, (int)sl, 64);
             if (in && dx != INF) {
                 const u64 nd = (u64)dx + a.y;
-                if (nd < INF) {
+                if (nd < INF && nd + h <= ub) {
                     const u64 cand = pack((u32)nd, px);
                     best = cand < best ? cand : best;
                 }
@@ -1095,7 +1109,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
         ls.edges += active ? deg : 0u;
         const bool imp = best < cur;
         cur = best;
-        active = imp && (u64)distOf(cur) < bnd;         // improved inside the bucket: propagates
+        active = imp && (u64)distOf(cur) + h < bnd;     // improved inside the bucket: propagates
     }
     // write back; lanes whose value is now the state's own expand their arcs leaving the tile
     bool expand = wasQueued;
@@ -1105,7 +1119,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
         const u64 old = atomicMin((unsigned long long*)&r.state[gi], (unsigned long long)cur);
         if (old > cur) {
             ls.updates++;
-            if ((u64)distOf(cur) < bnd) expand = true;
+            if ((u64)distOf(cur) + h < bnd) expand = true;
             else toFar = true;
         }
     }
@@ -1115,35 +1129,37 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     }
     // arcs leaving the tile, four per round, with the final distance
     Item it;
-    it.u = u; it.p = va.z; it.du = distOf(cur); it.k = k; it.ub = NOKEY; it.bnd = bnd;
+    it.u = u; it.p = va.z; it.du = distOf(cur); it.k = k; it.ub = ub; it.bnd = bnd;
     const u32 olen = expand ? deg : 0u;
     if (__ballot(olen != 0u)) {                         // the register arcs
-        Arcs4<false> x;
+        Arcs4<ASTAR> x;
 #pragma unroll
         for (u32 q = 0; q < 4; ++q) {
             x.ok[q] = q < olen && !inT[q];              // inside the tile: already relaxed
             x.a[q] = ra[q];
-            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = NOKEY; x.bnd[q] = bnd;
+            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = ub; x.bnd[q] = bnd;
+            if constexpr (ASTAR) x.t[q] = tg;
         }
-        relaxCore4<false, AM>(r, x, o, ls);
+        relaxCore4<ASTAR, AM>(r, x, o, ls);
     }
     const u32 omax = waveMax(olen);
     for (u32 off = TILE_ARCS; off < omax; off += 4) {
-        Arcs4<false> x;
+        Arcs4<ASTAR> x;
 #pragma unroll
         for (u32 q = 0; q < 4; ++q) {
             const u32 e = va.x + off + q;
             x.ok[q] = off + q < olen;
             x.a[q] = x.ok[q] ? r.g.adj[e] : make_uint2(0u, 0u);
             x.ok[q] = x.ok[q] && ((k * n + x.a[q].x) >> 6) != j;
-            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = NOKEY; x.bnd[q] = bnd;
+            x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = ub; x.bnd[q] = bnd;
+            if constexpr (ASTAR) x.t[q] = tg;
         }
-        relaxCore4<false, AM>(r, x, o, ls);
+        relaxCore4<ASTAR, AM>(r, x, o, ls);
     }
     if (expand && va.y < r1 && setBitFresh(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
 }
 
-template <int AM>
+template <bool ASTAR, int AM>
 __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls, u32* lbFar, u32* lbFarN, u32* lbR,
                                           u32* lbRN) {
     Ctl* c = r.ctl;
@@ -1194,7 +1210,7 @@ __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls
             if (tile != RING_EMPTY) ++my;
         }
         if (tile != RING_EMPTY) {
-            processTile<AM>(r, tile, o, lbR, lbRN, ls);
+            processTile<ASTAR, AM>(r, tile, o, lbR, lbRN, ls);
             // the wave's pushes and its slot reset are performed before the tile counts as finished
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");
@@ -1705,8 +1721,8 @@ __global__ void __launch_bounds__(1024) stepKernel(Run r, u32 persistent) {
                 break;
             }
             case PH_LIGHT_ASYNC:
-                if constexpr (!ASTAR && AM != AM_CAS_RBA) {
-                    if (r.tileMode) { tileLight<AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break; }
+                if constexpr (AM != AM_CAS_RBA) {
+                    if (r.tileMode) { tileLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN); break; }
                 }
                 asyncLight<ASTAR, AM>(r, fcur, ls, s_lbFar, &s_lbFarN, s_lbR, &s_lbRN);
                 break;

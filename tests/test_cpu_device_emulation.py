@@ -87,9 +87,13 @@ CASES = [
     ("grid", 24, "delta", "async=1 tile=1 tclaim=3 grid=4 pers=1 K=3 kmax=4 yield=2 seed=5"),
     ("grid", 30, "delta", "async=1 tile=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=2"),
     ("grid", 20, "delta", "async=1 tile=1 map=2 am=2 grid=4 pers=1 K=2 kmax=4 delta=300 yield=2"),
+    ("grid", 20, "delta", "async=1 tile=1 grid=3 pers=1 K=3 kmax=4 astar=1 yield=2"),
+    ("grid", 20, "delta", "async=1 tile=1 tclaim=2 grid=4 pers=1 K=3 kmax=4 astar=1 lm=8 delta=300 yield=3"),
+    ("grid", 24, "delta", "async=1 tile=1 am=2 grid=2 astar=1 delta=100000 fill=0"),
     # symmetric R-MAT: rows longer than the tiled phase's register arcs, self loops, parallel arcs
     ("rmat", 9, "delta", "async=1 tile=1 symm=1 tclaim=8 grid=3 pers=1 K=2 kmax=4 delta=60 yield=3"),
     ("rmat", 8, "delta", "async=1 tile=1 symm=1 am=2 grid=2 delta=30 fill=0 yield=2"),
+    ("rmat", 9, "delta", "async=1 tile=1 symm=1 grid=3 pers=1 K=2 kmax=4 astar=1 lm=4 delta=60 yield=2"),
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=3 seed=7"),
     ("rmat", 9, "delta", "async=1 grid=4 pers=1 K=2 kmax=4 delta=100000 ringcap=40 expect_full=1 yield=3"),
 ]

@@ -74,6 +74,21 @@ def test_alt_astar_matches_oracle(orc, grid, drive, algo):
             assert res["stats"].edges_scanned < eu["stats"].edges_scanned
 
 
+@pytest.mark.parametrize("async_light", [1, 2])
+def test_alt_astar_asynchronous_and_tiled(orc, grid, async_light):
+    """ALT with the asynchronous and the tiled light phase: same distances and paths."""
+    with P.GpuSolver(grid, max_batch=16) as sv:
+        sv.build_landmarks(8)
+        rng = np.random.default_rng(5)
+        srcs = rng.integers(0, grid.n, 16)
+        dsts = rng.integers(0, grid.n, 16)
+        res = sv.astar_batch(srcs, dsts, P.GpuOptions(drive=DR.Persistent, heuristic=HK.ALT, async_light=async_light))
+        for i in range(16):
+            d, path, _, _ = orc.astar(grid.row_offsets, grid.col_indices, grid.weights, grid.x, grid.y,
+                                      grid.h_scale, orc.H_FLOOR_EUCLID, int(srcs[i]), int(dsts[i]))
+            assert res["dist"][i] == d and np.array_equal(res["paths"][i], path), i
+
+
 def test_alt_on_directed_graph_and_given_landmarks(orc, rmat):
     with P.GpuSolver(rmat, max_batch=4) as sv:
         ids, sym = sv.build_landmarks(4, ids=[1, 2, 3, 4])

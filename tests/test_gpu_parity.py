@@ -395,6 +395,24 @@ def test_tiled_light_delta_values_and_batches(orc, grid100, sgrid100, delta):
         assert np.array_equal(ms.dist[k], d) and np.array_equal(ms.parent[k], pc), (delta, s)
 
 
+@pytest.mark.parametrize("heuristic", [HK.FloorEuclid, HK.Scaled], ids=lambda h: h.name)
+def test_tiled_light_astar(orc, grid100, sgrid100, heuristic):
+    """A* batches and single queries with the tiled light phase: same target distance and
+    canonical path as the oracle's A*."""
+    rng = np.random.default_rng(13)
+    srcs = rng.integers(0, grid100.n, 8)
+    dsts = rng.integers(0, grid100.n, 8)
+    for drive in (DR.Host, DR.Persistent):
+        o = P.GpuOptions(heuristic=heuristic, drive=drive, async_light=2)
+        res = sgrid100.astar_batch(srcs, dsts, o)
+        for i in range(8):
+            d, path, _, _ = oracle_astar(orc, grid100, srcs[i], dsts[i], heuristic)
+            assert res["dist"][i] == d and np.array_equal(res["paths"][i], path), (drive.name, i)
+        r = sgrid100.astar(int(srcs[0]), int(dsts[0]), o)
+        d, path, _, _ = oracle_astar(orc, grid100, srcs[0], dsts[0], heuristic)
+        assert r.target_dist == d and np.array_equal(r.path, path), drive.name
+
+
 @pytest.mark.parametrize("claim", [1, 3, 8, 64])
 def test_tiled_light_claims(orc, grid100, sgrid100, claim):
     """Waves claiming several queue slots at once (taken in order): same results."""
