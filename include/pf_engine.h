@@ -65,8 +65,8 @@ typedef struct {
     uint32_t async_light;  /* Delta-stepping: 1 = each bucket's light relaxations as one asynchronous
                               work-queue step (no grid barrier per light iteration), 0 = one step per
                               light iteration as the reference; 2 = asynchronous over 64-element tiles,
-                              each tile's light relaxations iterated in registers (SSSP on a graph whose
-                              arcs all have reverse arcs of equal weight; otherwise as 1); identical
+                              each tile's light relaxations iterated in registers (SSSP and A* on a graph
+                              whose arcs all have reverse arcs of equal weight; otherwise as 1); identical
                               results in every mode */
     uint32_t profile_phases; /* 1 = record wall-clock time per step phase (pf_solver_phase_profile) */
     uint32_t async_backoff;  /* asynchronous light phase: longest idle wait of a wave between polls of the

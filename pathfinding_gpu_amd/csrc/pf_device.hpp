@@ -118,7 +118,7 @@ struct Run {
     u32 async;                   // Delta-stepping with PH_LIGHT_ASYNC
     u32 profile;                 // accumulate wall-clock time per phase (controller, PROF_* words)
     u32 napCap;                  // async light phase: longest idle back-off, s_sleep(1) rounds
-    u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs, SSSP);
+    u32 tileMode;                // asynchronous light phase over tiles of 64 element ids (symmetric graphs);
                                  // a tile's near bits are one 64-bit word of nearMask[0], and a tile is queued
                                  // exactly while that word is non-zero (the push that makes it non-zero queues it)
     u32 tileClaim;               // tile mode: work-queue slots a wave claims per head update (>= 1)
@@ -1002,7 +1002,7 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
 
 // ---------------------------------------------------------------------------
 // Tiled asynchronous light phase (graphs whose every arc has a reverse arc of the same weight;
-// SSSP).  Work items are tiles of 64 consecutive element ids 64j .. 64j+63 (graphs with
+// SSSP and A*).  Work items are tiles of 64 consecutive element ids 64j .. 64j+63 (graphs with
 // coordinates are in Morton order, so one slot's tile is a compact patch).  A tile's 64 near bits
 // are one 64-bit word, and the tile is in the queue exactly while its word is non-zero: the near
 // mark that finds the word zero queues the tile.  A wave takes a tile: it fetches and clears the

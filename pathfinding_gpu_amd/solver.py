@@ -73,8 +73,8 @@ class GpuOptions:
     grid_blocks: int = 0
     max_steps: int = 0
     async_light: int = 0        # Delta: 1 = each bucket's light phase as one asynchronous work-queue step;
-                                # 2 = the same over 64-vertex tiles relaxed in registers (SSSP on symmetric
-                                # graphs; anything else runs as 1)
+                                # 2 = the same over 64-vertex tiles relaxed in registers (SSSP and A* on
+                                # symmetric graphs; anything else runs as 1)
     profile_phases: bool = False  # record wall-clock time per step phase (GpuSolver.phase_profile)
     async_backoff: int = 0      # async light phase: longest idle wait between queue polls (x64 clocks; 0 = 32)
     async_queue_slots: int = 0  # async light phase: work-queue slots (0 = 2*K*n); overflow reruns synchronously
