@@ -173,14 +173,16 @@ def test_multi_source_prefix_scan(orc, rmat12, srmat12):
         assert np.array_equal(r.dist[i], oracle_sssp(orc, rmat12, s)[0])
 
 
-def test_multi_seed_union_of_grids(orc):
+@pytest.mark.parametrize("async_light", [0, 1, 2])
+def test_multi_seed_union_of_grids(orc, async_light):
+    """One query seeded in each of 12 grids (several seeds may share a 64-vertex tile)."""
     rng = np.random.default_rng(8)
     gs = [P.grid_road(int(a), int(a), 1000 + i) for i, a in enumerate(rng.integers(8, 40, 12))]
     u, voff = P.union(gs)
     seeds = [int(voff[i]) + int(rng.integers(0, gs[i].n)) for i in range(len(gs))]
     with P.GpuSolver(u) as sv:
         for drive in (DR.Host, DR.Graph, DR.Persistent):
-            r = sv.multi_seed(seeds, P.GpuOptions(drive=drive))
+            r = sv.multi_seed(seeds, P.GpuOptions(drive=drive, async_light=async_light))
             for i, g in enumerate(gs):
                 sl = slice(int(voff[i]), int(voff[i + 1]))
                 d, pc = oracle_sssp(orc, g, seeds[i] - int(voff[i]))
