@@ -122,6 +122,7 @@ struct Run {
                                  // a tile's near bits are one 64-bit word of nearMask[0], and a tile is queued
                                  // exactly while that word is non-zero (the push that makes it non-zero queues it)
     u32 tileClaim;               // tile mode: work-queue slots a wave claims per head update (>= 1)
+    u32 tileKeep;                // tile mode: a wave takes the first tile it queues itself, at once
 };
 
 constexpr u32 BAR_GEN = 16;      // separate 128-byte line from the ticket
@@ -549,8 +550,11 @@ struct Arcs4 {
 // result is used (the caller issued the four arc loads together as well), so four arcs cost four
 // dependent round trips instead of sixteen, and the pushes take one reservation per queue for the
 // wave.
+// keepOne (tile mode): the wave's first newly queued tile is not pushed but returned (the same on
+// every lane; RING_EMPTY if there is none), for the caller to take it at once.
 template <bool ASTAR, int AM>
-__device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, const Outs& o, LocalStats& ls) {
+__device__ __forceinline__ u32 relaxCore4(const Run& r, const Arcs4<ASTAR>& x, const Outs& o, LocalStats& ls,
+                                          bool keepOne = false) {
     static_assert(AM != AM_CAS_RBA, "the CAS loop relaxes one arc at a time");
     const u32 n = r.g.n;
     bool go[4];
@@ -635,9 +639,22 @@ __device__ __forceinline__ void relaxCore4(const Run& r, const Arcs4<ASTAR>& x, 
         }
     }
     ls.pushes += nn + nf;
+    u32 kept = RING_EMPTY;
+    if (keepOne) {
+        const u64 has = __ballot(nn != 0);
+        if (has) {                                                      // wave-uniform
+            const int first = __ffsll((unsigned long long)has) - 1;
+            kept = __shfl(nearV[0], first, 64);
+            if ((int)laneId() == first) {                               // that lane's list loses its first item
+                nearV[0] = nearV[1]; nearV[1] = nearV[2]; nearV[2] = nearV[3];
+                --nn;
+            }
+        }
+    }
     const u64 pushing = __ballot(nn != 0);                              // the whole wave takes part
     if (o.ring && laneId() == 0 && pushing) ls.queueOps++;              // the wave's tail reservation
     pushGroup(r, o, nearV, nn, farV, nf);
+    return kept;
 }
 
 // One vertex's arcs [beg, end) spread over a wave or a workgroup: the caller's lane `me` of
@@ -1018,8 +1035,10 @@ __device__ __forceinline__ void asyncLight(const Run& r, u32 fcur, LocalStats& l
 // ---------------------------------------------------------------------------
 constexpr u32 TILE_ARCS = 4;   // light arcs per lane held in registers by the tiled phase (one Arcs4 round)
 
+// Returns a tile this call queued and kept for the caller to take next (RING_EMPTY if none).
 template <bool ASTAR, int AM>
-__device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, u32* lbR, u32* lbRN, LocalStats& ls) {
+__device__ __forceinline__ u32 processTile(const Run& r, u32 j, const Outs& o, u32* lbR, u32* lbRN, LocalStats& ls,
+                                           bool keepOne) {
     const u32 n = r.g.n;
     const u32 lane = laneId();
     const u32 gi = j * 64u + lane;
@@ -1131,6 +1150,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
     Item it;
     it.u = u; it.p = va.z; it.du = distOf(cur); it.k = k; it.ub = ub; it.bnd = bnd;
     const u32 olen = expand ? deg : 0u;
+    u32 kept = RING_EMPTY;
     if (__ballot(olen != 0u)) {                         // the register arcs
         Arcs4<ASTAR> x;
 #pragma unroll
@@ -1140,7 +1160,7 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
             x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = ub; x.bnd[q] = bnd;
             if constexpr (ASTAR) x.t[q] = tg;
         }
-        relaxCore4<ASTAR, AM>(r, x, o, ls);
+        kept = relaxCore4<ASTAR, AM>(r, x, o, ls, keepOne);
     }
     const u32 omax = waveMax(olen);
     for (u32 off = TILE_ARCS; off < omax; off += 4) {
@@ -1154,9 +1174,11 @@ __device__ __forceinline__ void processTile(const Run& r, u32 j, const Outs& o, 
             x.du[q] = it.du; x.k[q] = k; x.p[q] = it.p; x.ub[q] = ub; x.bnd[q] = bnd;
             if constexpr (ASTAR) x.t[q] = tg;
         }
-        relaxCore4<ASTAR, AM>(r, x, o, ls);
+        const u32 k2 = relaxCore4<ASTAR, AM>(r, x, o, ls, keepOne && kept == RING_EMPTY);
+        kept = kept == RING_EMPTY ? k2 : kept;
     }
     if (expand && va.y < r1 && setBitFresh(r.rMask, gi)) pushBuf(lbR, lbRN, LB_R, r.rQ, &r.ctl->cnt[C_R], gi);
+    return kept;
 }
 
 template <bool ASTAR, int AM>
@@ -1210,7 +1232,8 @@ __device__ __forceinline__ void tileLight(const Run& r, u32 fcur, LocalStats& ls
             if (tile != RING_EMPTY) ++my;
         }
         if (tile != RING_EMPTY) {
-            processTile<ASTAR, AM>(r, tile, o, lbR, lbRN, ls);
+            // a tile this wave queued and kept is taken at once, without a trip through the queue
+            for (u32 t = tile; t != RING_EMPTY;) t = processTile<ASTAR, AM>(r, t, o, lbR, lbRN, ls, r.tileKeep != 0);
             // the wave's pushes and its slot reset are performed before the tile counts as finished
             __builtin_amdgcn_wave_barrier();
             asm volatile("" ::: "memory");

@@ -5,7 +5,7 @@
 // hipMalloc leaves them, and checks dist and parent (A*: target distance and canonical path)
 // against the oracle.  Prints "ok: ..." or "FAIL: ...".
 //   emu g=grid|rmat size= algo=naive|frontier|scan|delta map= am= bs= grid= K= kmax= delta=
-//       pers= astar= lm= sym= src= yield= fill= seed= symm= (R-MAT with reverse arcs) tile= tclaim=
+//       pers= astar= lm= sym= src= yield= fill= seed= symm= (R-MAT with reverse arcs) tile= tclaim= tkeep=
 #include "shim.h"
 #include "dev_emu_0.hpp"   // one copy of the device code per concurrent workgroup: the copies'
 #include "dev_emu_1.hpp"   // function-scope statics stand for each workgroup's LDS
@@ -176,6 +176,7 @@ int main(int argc, char** argv) {
         if (gi_("tile", 0)) {   // tiled variant (symmetric graphs: the grid generator's)
             r.tileMode = 1;
             r.tileClaim = (u32)gi_("tclaim", 1);
+            r.tileKeep = (u32)gi_("tkeep", 0);
         }
     }
     if (scan) { startPhase = PH_SCAN_RELAX; useBound = 0; }

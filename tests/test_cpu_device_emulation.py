@@ -83,6 +83,10 @@ CASES = [
     ("grid", 30, "delta", "async=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=1"),
     # tiled asynchronous light phase (64-element tiles relaxed in registers)
     ("grid", 20, "delta", "async=1 tile=1 grid=3"),
+    ("grid", 20, "delta", "async=1 tile=1 tkeep=1 grid=3"),
+    ("grid", 30, "delta", "async=1 tile=1 tkeep=1 grid=4 pers=1 K=3 kmax=4 yield=3 seed=3"),
+    ("grid", 20, "delta", "async=1 tile=1 tkeep=1 grid=3 pers=1 K=2 kmax=4 astar=1 lm=8 yield=2"),
+    ("rmat", 9, "delta", "async=1 tile=1 tkeep=1 symm=1 grid=3 pers=1 delta=60 yield=2"),
     ("grid", 24, "delta", "async=1 tile=1 grid=4 pers=1 K=3 kmax=4 yield=3"),
     ("grid", 24, "delta", "async=1 tile=1 tclaim=3 grid=4 pers=1 K=3 kmax=4 yield=2 seed=5"),
     ("grid", 30, "delta", "async=1 tile=1 grid=4 pers=1 delta=100000 yield=4 fill=0 seed=2"),
