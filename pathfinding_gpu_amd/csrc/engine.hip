@@ -487,7 +487,9 @@ int doRun(pf_solver* s, RunSpec& spec, pf_stats* st) {
             // one slot per claim for a single query (latency-bound: no tile may wait behind a busy
             // wave); batches queue many more tiles, and one head counter serializes the claims
             run.tileClaim = o.async_tile_claim ? std::min<u32>(o.async_tile_claim, 64u) : (spec.k > 1 ? 4u : 1u);
-            run.tileKeep = spec.k == 1 ? 1u : 0u;   // one query: follow the front locally
+            // a wave takes the first tile it queues itself at once: faster for one query and for A*
+            // batches (their fronts are narrow), slower for SSSP batches
+            run.tileKeep = spec.k == 1 || spec.astar ? 1u : 0u;
         }
     }
     run.profile = o.profile_phases ? 1u : 0u;
